@@ -1,0 +1,235 @@
+"""ctypes binding of the CPU f64 oracle — TEST INFRASTRUCTURE ONLY.
+
+Loaded by the test suite, the smoke check and the benchmark's CPU-baseline
+leg, always as the checker.  The product package never imports this module.
+Trees are anything with a `.preorder()` method returning (ops, vals) arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "librcsr_oracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(_LIB_PATH)
+        L.or_eval.restype = C.c_double
+        L.or_eval.argtypes = [_ip, _dp, C.c_int, _dp, C.c_int]
+        L.or_erf.restype = C.c_double
+        L.or_erf.argtypes = [C.c_double]
+        L.or_erfc.restype = C.c_double
+        L.or_erfc.argtypes = [C.c_double]
+        L.or_nconst.restype = C.c_int
+        L.or_nconst.argtypes = [_ip, _dp, C.c_int]
+        L.or_sse.restype = C.c_double
+        L.or_sse.argtypes = [_ip, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp]
+        L.or_gradients.restype = None
+        L.or_gradients.argtypes = [_ip, _dp, C.c_int, _dp, C.c_int, _dp, _dp]
+        L.or_fit_weighted.restype = C.c_double
+        L.or_fit_weighted.argtypes = [_ip, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp,
+                                      C.c_int, _dp]
+        L.or_fit_metric.restype = C.c_double
+        L.or_fit_metric.argtypes = [_ip, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp,
+                                    C.c_int, C.c_int, _dp]
+        L.or_metric_score.restype = C.c_double
+        L.or_metric_score.argtypes = [_ip, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp,
+                                      C.c_int]
+        L.or_implicit_score.restype = C.c_double
+        L.or_implicit_score.argtypes = [_ip, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp, C.c_int]
+        L.or_signed_distance.restype = C.c_int
+        L.or_signed_distance.argtypes = [_ip, _dp, C.c_int, _dp, C.c_int, _dp]
+        L.or_implicit_fit.restype = C.c_double
+        L.or_implicit_fit.argtypes = [_ip, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp, C.c_int,
+                                      _dp]
+        L.or_fit_batch.restype = None
+        L.or_fit_batch.argtypes = [_ip, _dp, _ip, C.c_int, _ip, _dp, C.c_int, C.c_int, _dp,
+                                   _dp, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.or_implicit_fit_batch.restype = None
+        L.or_implicit_fit_batch.argtypes = [_ip, _dp, _ip, C.c_int, _dp, C.c_int, C.c_int, _dp,
+                                            C.c_int, C.c_int, C.c_int, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _tree(t) -> Tuple[np.ndarray, np.ndarray]:
+    ops, vals = t.preorder()
+    return np.ascontiguousarray(ops, np.int32), np.ascontiguousarray(vals, np.float64)
+
+
+def _rows(x) -> np.ndarray:
+    x = np.asarray(x, dtype=np.float64)
+    if x.ndim == 1:
+        x = x[:, None]
+    return np.ascontiguousarray(x)
+
+
+def _w(w) -> Optional[np.ndarray]:
+    if w is None or len(w) == 0:
+        return None
+    return np.ascontiguousarray(w, dtype=np.float64)
+
+
+def eval_row(tree, x) -> float:
+    ops, vals = _tree(tree)
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().or_eval(_i(ops), _d(vals), len(ops), _d(x), len(x))
+
+
+def predict(tree, X) -> np.ndarray:
+    X = _rows(X)
+    return np.array([eval_row(tree, X[i]) for i in range(X.shape[0])])
+
+
+def erf(x: float) -> float:
+    return lib().or_erf(float(x))
+
+
+def erfc(x: float) -> float:
+    return lib().or_erfc(float(x))
+
+
+def nconst(tree) -> int:
+    ops, vals = _tree(tree)
+    return lib().or_nconst(_i(ops), _d(vals), len(ops))
+
+
+def sse(tree, X, y, w=None) -> float:
+    ops, vals = _tree(tree)
+    X = _rows(X)
+    y = np.ascontiguousarray(y, np.float64)
+    ww = _w(w)
+    return lib().or_sse(_i(ops), _d(vals), len(ops), _d(X), X.shape[0], X.shape[1], _d(y), _d(ww))
+
+
+def gradients(tree, x) -> Tuple[np.ndarray, np.ndarray]:
+    ops, vals = _tree(tree)
+    x = np.ascontiguousarray(x, np.float64)
+    k = nconst(tree)
+    g = np.zeros(max(k, 1))
+    gx = np.zeros(max(len(x), 1))
+    lib().or_gradients(_i(ops), _d(vals), len(ops), _d(x), len(x), _d(g), _d(gx))
+    return g[:k], gx[:len(x)]
+
+
+def fit_weighted(tree, X, y, w=None, iters=8) -> Tuple[float, np.ndarray]:
+    ops, vals = _tree(tree)
+    X = _rows(X)
+    y = np.ascontiguousarray(y, np.float64)
+    ww = _w(w)
+    c = np.zeros(max(nconst(tree), 1))
+    s = lib().or_fit_weighted(_i(ops), _d(vals), len(ops), _d(X), X.shape[0], X.shape[1], _d(y),
+                              _d(ww), int(iters), _d(c))
+    return s, c[:nconst(tree)]
+
+
+def fit_metric(tree, X, y, w=None, iters=8, metric=0) -> Tuple[float, np.ndarray]:
+    ops, vals = _tree(tree)
+    X = _rows(X)
+    y = np.ascontiguousarray(y, np.float64)
+    ww = _w(w)
+    c = np.zeros(max(nconst(tree), 1))
+    s = lib().or_fit_metric(_i(ops), _d(vals), len(ops), _d(X), X.shape[0], X.shape[1], _d(y),
+                            _d(ww), int(iters), int(metric), _d(c))
+    return s, c[:nconst(tree)]
+
+
+def metric_score(tree, X, y, w=None, metric=0) -> float:
+    ops, vals = _tree(tree)
+    X = _rows(X)
+    y = np.ascontiguousarray(y, np.float64)
+    ww = _w(w)
+    return lib().or_metric_score(_i(ops), _d(vals), len(ops), _d(X), X.shape[0], X.shape[1],
+                                 _d(y), _d(ww), int(metric))
+
+
+def implicit_score(tree, X, w=None, metric=0) -> float:
+    ops, vals = _tree(tree)
+    X = _rows(X)
+    ww = _w(w)
+    return lib().or_implicit_score(_i(ops), _d(vals), len(ops), _d(X), X.shape[0], X.shape[1],
+                                   _d(ww), int(metric))
+
+
+def signed_distance(tree, x) -> Optional[float]:
+    ops, vals = _tree(tree)
+    x = np.ascontiguousarray(x, np.float64)
+    d = np.zeros(1)
+    rc = lib().or_signed_distance(_i(ops), _d(vals), len(ops), _d(x), len(x), _d(d))
+    return None if rc != 0 else float(d[0])
+
+
+def implicit_fit(tree, X, w=None, iters=8) -> Tuple[float, np.ndarray]:
+    ops, vals = _tree(tree)
+    X = _rows(X)
+    ww = _w(w)
+    k = nconst(tree)
+    c = np.zeros(max(k, 1))
+    s = lib().or_implicit_fit(_i(ops), _d(vals), len(ops), _d(X), X.shape[0], X.shape[1], _d(ww),
+                              int(iters), _d(c))
+    return s, c[:k]
+
+
+def fit_batch(trees: Sequence, X, y, w=None, iters=8, metrics=None, threads=1, kmax=20):
+    """Fit many trees on the host (one tree per thread)."""
+    ops_l, vals_l, off = [], [], [0]
+    for t in trees:
+        o, v = _tree(t)
+        ops_l.append(o)
+        vals_l.append(v)
+        off.append(off[-1] + len(o))
+    ops = np.ascontiguousarray(np.concatenate(ops_l), np.int32)
+    vals = np.ascontiguousarray(np.concatenate(vals_l), np.float64)
+    off = np.asarray(off, np.int32)
+    X = _rows(X)
+    y = np.ascontiguousarray(y, np.float64)
+    ww = _w(w)
+    met = None if metrics is None else np.ascontiguousarray(metrics, np.int32)
+    n = len(trees)
+    out_c = np.zeros((n, kmax))
+    out_l = np.zeros(n)
+    lib().or_fit_batch(_i(ops), _d(vals), _i(off), n, None if met is None else _i(met), _d(X),
+                       X.shape[0], X.shape[1], _d(y), _d(ww), int(iters), int(kmax),
+                       int(threads), _d(out_c), _d(out_l))
+    return out_l, out_c
+
+
+def implicit_fit_batch(trees: Sequence, X, w=None, iters=8, threads=1, kmax=8):
+    ops_l, vals_l, off = [], [], [0]
+    for t in trees:
+        o, v = _tree(t)
+        ops_l.append(o)
+        vals_l.append(v)
+        off.append(off[-1] + len(o))
+    ops = np.ascontiguousarray(np.concatenate(ops_l), np.int32)
+    vals = np.ascontiguousarray(np.concatenate(vals_l), np.float64)
+    off = np.asarray(off, np.int32)
+    X = _rows(X)
+    ww = _w(w)
+    n = len(trees)
+    out_c = np.zeros((n, kmax))
+    out_l = np.zeros(n)
+    lib().or_implicit_fit_batch(_i(ops), _d(vals), _i(off), n, _d(X), X.shape[0], X.shape[1],
+                                _d(ww), int(iters), int(kmax), int(threads), _d(out_c),
+                                _d(out_l))
+    return out_l, out_c
