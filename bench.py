@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the rcsr device arm on one MI355X.
+
+Workload (one "step"): one batched LM fit of 1024 candidates with
+k ~ U{1..20} constants over N = 262144 rows of 3 inputs, 30 LM iterations
+plus the exact f64 re-baseline and 4-step polish, through the C ABI
+(rc_fit_batch) with every input already resident in HBM.  Metric: LM-fitted
+candidates per second of the mixed-precision arm (FP32 interpreter and
+adjoint, FP32-MFMA split Gram, FP64 solve and acceptance).
+
+Beside it, in the same run:
+  - the plain FP64 device arm on the same batch (the speed-up denominator);
+  - the fitted-loss relative error of the mixed arm against the FP64 device
+    arm (all candidates) and against the CPU f64 oracle (a bounded sample);
+  - the CPU f64 oracle (a C++ restatement of the reference's fit_lm) timed on
+    the host cores on a bounded sample of the same batch;
+  - the assembly kernel's roofline: algorithmic flops 2 N (k^2 + k) per
+    candidate per assembly over the measured assembly time, against the FP32
+    MFMA peak.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "the-rule-and-compass_amd"))
+
+METRIC = "LM-fitted candidates/s at N=262144 (mixed-prec+adjoint vs plain f64); row-evals/s"
+FP32_MFMA_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=262144)
+    ap.add_argument("--cands", type=int, default=1024)
+    ap.add_argument("--iters", type=int, default=30)
+    ap.add_argument("--fp64-cands", type=int, default=-1,
+                    help="candidates in the FP64-arm timing (-1: all)")
+    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box in this setup has one MI355X; "
+              f"--gpus {args.gpus} is not supported (candidate sharding needs no "
+              f"communication, see the design notes).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from rcsr_hip import host, lib, workloads
+
+    t0 = time.time()
+    wl = workloads.config3(n=args.n, ncand=args.cands, iters=args.iters)
+    plans = [host.gpu_fit_plan(t) for t in wl.trees]
+    assert all(p is not None for p in plans)
+    ks = np.asarray([p.nconst for p in plans])
+    log(f"workload ready in {time.time() - t0:.1f}s: {len(plans)} candidates, N={wl.n}, "
+        f"k mean {ks.mean():.2f}, nodes mean {np.mean([p.nodes for p in plans]):.1f}")
+
+    ev = host.GpuEvaluator(wl.cols32(), wl.n, wl.v, wl.y.astype(np.float32))
+    assert ev.set_fit_data(wl.cols64(), wl.y)
+    L = lib.load()
+
+    def fit():
+        outs, ms = ev.fit_batch(plans, args.iters, 0, kmax=20)
+        if ms < 0:
+            raise RuntimeError("rc_fit_batch failed")
+        return outs, ms
+
+    for w in range(args.warmup):
+        ts = time.perf_counter()
+        outs, ms = fit()
+        log(f"warmup {w}: {time.perf_counter() - ts:.3f}s (device {ms:.1f} ms)")
+    times = []
+    for s in range(args.steps):
+        ts = time.perf_counter()
+        outs, ms = fit()
+        times.append(time.perf_counter() - ts)
+        log(f"step {s}: {times[-1]:.3f}s (device {ms:.1f} ms)")
+    step_s = float(np.mean(times))
+    value = len(plans) / step_s
+    statuses = np.asarray([o.status for o in outs])
+    mixed_loss = np.asarray([o.sse for o in outs])
+
+    # Per-pass timing of one extra (untimed) fit for the roofline.
+    import ctypes as C
+    L.rc_fit_profile(ev.ctx, 1)
+    fit()
+    rep = lib.RcFitReport()
+    per = np.zeros(3 * len(plans), np.int32)
+    L.rc_fit_report(ev.ctx, C.byref(rep), lib.ptr(per, lib.i32p), len(plans))
+    L.rc_fit_profile(ev.ctx, 0)
+    per = per.reshape(-1, 3)
+    flops = float(np.sum(per[:, 0] * 2.0 * wl.n * (ks.astype(np.float64) ** 2 + ks)))
+    asm_tflops = flops / (rep.asm_ms * 1e-3) / 1e12 if rep.asm_ms > 0 else 0.0
+    # Exact-loss pass bytes: cols64 (3 x 8 B) + y64 (8 B) per row per candidate pass.
+    log(f"profile: asm {rep.asm_ms:.1f} ms / {rep.asm_launches} passes, trial {rep.trial_ms:.1f} ms "
+        f"/ {rep.trial_launches}, exact {rep.exact_ms:.1f} ms / {rep.exact_launches}, ctrl "
+        f"{rep.ctrl_ms:.1f} ms / {rep.ctrl_launches}; assemblies/cand {per[:, 0].mean():.1f}")
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "candidates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 rows + f64 solve/acceptance",
+        "data": "synthetic (Philox-seeded C3 shape)",
+        "config": {
+            "workload": wl.name,
+            "candidates": len(plans),
+            "n_rows": wl.n,
+            "inputs": wl.v,
+            "lm_iterations": args.iters,
+            "k_mean": float(ks.mean()),
+            "metric": "squared",
+        },
+        "roofline": {
+            "bound": "mfma",
+            "kernel": "ft_asm_kernel (forward + adjoint + FP32-MFMA Gram)",
+            "achieved": asm_tflops,
+            "peak": FP32_MFMA_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": asm_tflops / FP32_MFMA_PEAK_TFLOPS,
+            "traffic": None,
+            "asm_ms_per_fit": rep.asm_ms,
+            "asm_launches": rep.asm_launches,
+            "trial_ms_per_fit": rep.trial_ms,
+            "exact_ms_per_fit": rep.exact_ms,
+            "ctrl_ms_per_fit": rep.ctrl_ms,
+        },
+        "status_counts": {str(int(s)): int(np.sum(statuses == s)) for s in np.unique(statuses)},
+    }
+
+    # The plain FP64 device arm on the same batch.
+    if not args.no_fp64:
+        nsub = len(plans) if args.fp64_cands < 0 else min(args.fp64_cands, len(plans))
+        ev.set_fit_mode(host.FIT_MODE_FP64)
+        ts = time.perf_counter()
+        f64outs, ms64 = ev.fit_batch(plans[:nsub], args.iters, 0, kmax=20)
+        t64 = time.perf_counter() - ts
+        ev.set_fit_mode(host.FIT_MODE_MIXED)
+        if ms64 < 0:
+            raise RuntimeError("FP64 arm failed")
+        v64 = nsub / t64
+        f64loss = np.asarray([o.sse for o in f64outs])
+        okm = np.array([host.fit_status_usable(outs[i].status) and outs[i].status != 7 and
+                        host.fit_status_usable(f64outs[i].status) and f64outs[i].status != 7
+                        and np.isfinite(f64loss[i]) and f64loss[i] > 0 for i in range(nsub)])
+        rel = np.abs(mixed_loss[:nsub][okm] - f64loss[okm]) / f64loss[okm]
+        log(f"fp64 arm: {nsub} candidates in {t64:.2f}s -> {v64:.2f} cand/s; mixed vs fp64 "
+            f"loss rel err median {np.median(rel):.2e}, p99 {np.quantile(rel, 0.99):.2e}, "
+            f"max {np.max(rel):.2e}")
+        result["fp64_device"] = {"value": v64, "unit": "candidates/s", "candidates": nsub,
+                                 "seconds": t64}
+        result["speedup_vs_fp64"] = value / v64
+        result["loss_rel_err_vs_fp64_device"] = {
+            "median": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)),
+            "max": float(np.max(rel)), "frac_le_1e-6": float(np.mean(rel <= 1e-6)),
+            "compared": int(okm.sum())}
+
+    # The CPU f64 oracle on a bounded sample of the same batch.
+    if not args.no_cpu and args.cpu_sample > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = min(os.cpu_count() or 1, 16)
+        ns = min(args.cpu_sample, len(plans))
+        ts = time.perf_counter()
+        cl, _ = oracle.fit_batch(wl.trees[:ns], wl.x, wl.y, None, args.iters, threads=threads,
+                                 kmax=20)
+        tc = time.perf_counter() - ts
+        ok = np.isfinite(cl) & (cl > 0) & np.array([outs[i].status != 7 for i in range(ns)])
+        rel = np.abs(mixed_loss[:ns][ok] - cl[ok]) / cl[ok]
+        log(f"cpu oracle: {ns} candidates on {threads} threads in {tc:.1f}s; mixed vs oracle "
+            f"median {np.median(rel):.2e} max {np.max(rel):.2e}")
+        result["cpu_baseline"] = {
+            "value": ns / tc, "unit": "candidates/s", "cores": threads, "kind": "port",
+            "sample": f"first {ns} of the {len(plans)} candidates, full N={wl.n}, "
+                      f"{args.iters} LM iterations + 4 polish, one candidate per thread"}
+        result["loss_rel_err_vs_cpu_oracle"] = {
+            "median": float(np.median(rel)), "max": float(np.max(rel)),
+            "compared": int(ok.sum())}
+    ev.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,33 @@
+/*
+ * Host-side program builders exported by librcsrhip.so (pure CPU code).
+ *
+ * They restate the reference's Rust host layer that sits directly above the
+ * device ABI, so a C/C++/Python caller can produce exactly the programs the
+ * rc_* entry points consume.  Trees are pre-order node lists: ops[i] is an
+ * operator code (device opcodes 0..42, plus 43 asinh, 44 acosh, 45 atanh),
+ * vals[i] the constant value (CONST) or column index (VAR).
+ */
+#ifndef RCSR_HOST_H
+#define RCSR_HOST_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* gpu.rs:198-218 compile_program (lower, peak-stack gate at 56, emit, HALT).
+   Returns the word count, or -1 unsupported operator, -2 stack too deep
+   (*peak holds the depth), -3 malformed tree or buffer too small. */
+int rch_compile_program(const int *ops, const double *vals, int nn, unsigned int *out, int cap,
+                        int *peak);
+
+/* fit.rs:985-1056 gpu_fit_plan_reason: post-order 4-word records and start
+   constants.  Returns the constant count, or -1 no constants, -2 more than 20
+   constants, -3 more than 96 nodes, -5 unsupported operator, -6 malformed
+   tree or buffer too small. */
+int rch_fit_plan(const int *ops, const double *vals, int nn, unsigned int *out_words,
+                 int cap_words, double *out_c0, int cap_c, int *nodes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,799 @@
+/*
+ * Batched expression-bytecode evaluation on gfx950.
+ *
+ * Reference behaviour: rc_exec_row (eval.cu:404-573), eval_kernel /
+ * eval_pool_kernel (759-828), score_kernel modes 0-4, 7-9 (846-1088),
+ * predict_kernel (1093-1104) and the ABI at 3268-3450, 3514-3600, 3803-3880.
+ *
+ * MI355X design (not the reference's block-per-program layout):
+ *  - the unit of work is one wavefront on one (program, row tile); a launch
+ *    has nprog x tiles single-wave workgroups, so 4096 programs x 1M rows is
+ *    ~1M waves and every CU stays busy regardless of the program mix;
+ *  - the program stream is wave-uniform: instruction words come through the
+ *    scalar cache (s_load), the operator switch is a uniform branch and the
+ *    operand stack's depth is an SGPR, so the stack lives in VGPRs addressed
+ *    with s_set_gpr_idx (no scratch, no LDS).  The top of stack is cached in
+ *    a fixed register so a binary operator costs one indexed read;
+ *  - each lane evaluates R rows per pass (rows base + r*64 + lane), which
+ *    amortises the scalar dispatch over R rows and keeps every column read a
+ *    coalesced 256-byte wave access;
+ *  - workgroups are mapped XCD-aware: workgroup g runs on XCD g % 8, and XCD x
+ *    is given the x-th eighth of the row tiles for every program, so a tile of
+ *    rows is pulled into one XCD's L2 once and reused by all programs;
+ *  - stack errors (underflow, overflow past 64, missing HALT) are static
+ *    properties of a program, so the host checks them once and replaces an
+ *    erroneous program by a NaN program; the kernel carries no per-op guard.
+ *  - partial sums are per (program, tile) in f64 with a fixed wave butterfly,
+ *    finished by a per-program pass in tile order: results depend only on the
+ *    program and n, never on batch composition.
+ */
+#include "rc_device.h"
+#include "rc_internal.h"
+
+#include <string.h>
+
+#include <cmath>
+
+#include <vector>
+
+namespace {
+
+/* Evaluator internal stream: OP_CONST w, f32 bits | OP_VAR | (col << 16) |
+   operator words | OP_HALT.  Unknown operators become RC_OP_NAN (sets the top
+   of stack to NaN, the reference's unary default). */
+constexpr uint32_t RC_OP_NAN = 254;
+
+constexpr int MODE_EVAL = 0;    /* sum of rho with 1e30 hard failures (rc_eval) */
+constexpr int MODE_SCORE = 1;   /* f64 score objectives + bad flag */
+constexpr int MODE_PREDICT = 2; /* raw predictions */
+
+bool is_binary_op(uint32_t op)
+{
+    return (op >= OP_ADD && op <= OP_DIV) || op == OP_POW || op == OP_MOD || op == OP_ATAN2 ||
+           op == OP_MIN || op == OP_MAX || (op >= OP_EQUAL && op <= OP_XOR);
+}
+
+bool is_unary_op(uint32_t op)
+{
+    return (op >= OP_SIN && op <= OP_GAUSS && op != OP_POW) || op == OP_STEP || op == OP_SINH ||
+           op == OP_COSH || op == OP_FLOOR || op == OP_CEIL || op == OP_ROUND || op == OP_ERF ||
+           op == OP_ERFC || op == OP_SIGN || op == OP_NOT;
+}
+
+/* Translate one program; returns its peak depth, or -1 for a program the
+   reference device would score as NaN on every row. */
+int translate_one(const uint32_t *w, size_t len, int nvars, std::vector<uint32_t> &out)
+{
+    int depth = 0, peak = 0;
+    size_t i = 0;
+    size_t start = out.size();
+    for (;;) {
+        if (i >= len) { out.resize(start); return -1; } /* no HALT in range */
+        uint32_t op = w[i++];
+        if (op == OP_CONST || op == OP_VAR) {
+            if (i >= len) { out.resize(start); return -1; }
+            uint32_t arg = w[i++];
+            if (depth >= RC_MAX_STACK) { out.resize(start); return -1; }
+            ++depth;
+            peak = depth > peak ? depth : peak;
+            if (op == OP_CONST) {
+                out.push_back(OP_CONST);
+                out.push_back(arg);
+            } else if ((int)arg < nvars && arg < 65536u) {
+                out.push_back(OP_VAR | (arg << 16));
+            } else {
+                /* A column outside the resident set: NaN, not a stray read. */
+                out.push_back(OP_CONST);
+                out.push_back(0x7fc00000u);
+            }
+        } else if (op == OP_HALT) {
+            if (depth <= 0) { out.resize(start); return -1; }
+            out.push_back(OP_HALT);
+            return peak;
+        } else if (op == OP_IF) {
+            if (depth < 3) { out.resize(start); return -1; }
+            depth -= 2;
+            out.push_back(OP_IF);
+        } else if (is_binary_op(op)) {
+            if (depth < 2) { out.resize(start); return -1; }
+            depth -= 1;
+            out.push_back(op);
+        } else {
+            if (depth < 1) { out.resize(start); return -1; }
+            out.push_back(is_unary_op(op) ? op : RC_OP_NAN);
+        }
+    }
+}
+
+} /* namespace */
+
+/* Translate a batch.  meta = [code offset per program | program list grouped
+   by stack class]; cls_start receives the class boundaries in that list. */
+int rc_translate_programs(const uint32_t *prog, const uint32_t *off, int nprog, int nvars,
+                          std::vector<uint32_t> &code, std::vector<uint32_t> &meta,
+                          int cls_start[5])
+{
+    code.clear();
+    meta.assign((size_t)nprog * 2, 0);
+    std::vector<int> depth((size_t)nprog, 1);
+    int peak = 1;
+    for (int p = 0; p < nprog; ++p) {
+        meta[p] = (uint32_t)code.size();
+        size_t a = off[p], b = off[p + 1];
+        int d = (b > a) ? translate_one(prog + a, b - a, nvars, code) : -1;
+        if (d < 0) {
+            code.push_back(OP_CONST);
+            code.push_back(0x7fc00000u);
+            code.push_back(OP_HALT);
+            d = 1;
+        }
+        depth[p] = d;
+        peak = d > peak ? d : peak;
+    }
+    static const int lim[4] = {8, 16, 32, 64};
+    int k = 0;
+    for (int cls = 0; cls < 4; ++cls) {
+        cls_start[cls] = k;
+        for (int p = 0; p < nprog; ++p) {
+            const int d = depth[p];
+            const int mine = d <= 8 ? 0 : d <= 16 ? 1 : d <= 32 ? 2 : 3;
+            if (mine == cls) meta[(size_t)nprog + k++] = (uint32_t)p;
+        }
+        (void)lim;
+    }
+    cls_start[4] = k;
+    return peak;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Device interpreter.                                                       */
+
+__device__ __noinline__ float rc_unary_call(unsigned op, float a) { return rc_unary_f32(op, a); }
+__device__ __noinline__ float rc_binary_call(unsigned op, float a, float b)
+{
+    return rc_binary_f32(op, a, b);
+}
+
+/* Evaluate the program at `pc` for R rows of this lane.  `xrow[r]` is the
+   row index (clamped in range by the caller). */
+template <int S, int R>
+__device__ __forceinline__ void rc_run(const uint32_t *__restrict__ pc,
+                                       const float *__restrict__ cols, int n,
+                                       const int *xrow, float *res)
+{
+    /* st[r][d]: the compiler keeps this in VGPRs (indexed by the uniform
+       depth) as long as R * S <= 32; the per-class (S, R) choice keeps it so. */
+    float tos[R];
+    float st[R][S];
+    int sp = 0;      /* entries below the top of stack */
+    int depth = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) tos[r] = 0.0f;
+    for (;;) {
+        const uint32_t w = *pc++;
+        const uint32_t op = w & 0xffu;
+        if (op == OP_VAR || op == OP_CONST) {
+            if (depth > 0) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) st[r][sp] = tos[r];
+                ++sp;
+            }
+            ++depth;
+            if (op == OP_VAR) {
+                const float *col = cols + (size_t)(w >> 16) * (size_t)n;
+#pragma unroll
+                for (int r = 0; r < R; ++r) tos[r] = col[xrow[r]];
+            } else {
+                const float c = __uint_as_float(*pc++);
+#pragma unroll
+                for (int r = 0; r < R; ++r) tos[r] = c;
+            }
+        } else if (op == OP_MUL) {
+            --sp; --depth;
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = st[r][sp] * tos[r];
+        } else if (op == OP_ADD) {
+            --sp; --depth;
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = st[r][sp] + tos[r];
+        } else if (op == OP_SUB) {
+            --sp; --depth;
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = st[r][sp] - tos[r];
+        } else if (op == OP_DIV) {
+            --sp; --depth;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const float b = tos[r];
+                tos[r] = (fabsf(b) < 1e-12f) ? RC_F32_NAN : st[r][sp] / b;
+            }
+        } else if (op == OP_HALT) {
+            break;
+        } else if (op == OP_NEG) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = -tos[r];
+        } else if (op == OP_SQUARE) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = tos[r] * tos[r];
+        } else if (op == OP_IF) {
+            sp -= 2; depth -= 2;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const float cnd = st[r][sp];
+                const float thn = st[r][sp + 1];
+                tos[r] = (cnd > 0.0f) ? thn : tos[r];
+            }
+        } else if (op == OP_EXP) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = expf(tos[r]);
+        } else if (op == OP_LOG) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = (tos[r] > 0.0f) ? logf(tos[r]) : RC_F32_NAN;
+        } else if (op == OP_SIN) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = sinf(tos[r]);
+        } else if (op == OP_COS) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = cosf(tos[r]);
+        } else if (op == OP_POW || op == OP_MOD || op == OP_ATAN2 || op == OP_MIN ||
+                   op == OP_MAX || (op >= OP_EQUAL && op <= OP_XOR)) {
+            /* Rare binary operators: one out-of-line call per row keeps the
+               interpreter's code small; the indices stay static. */
+            --sp; --depth;
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = rc_binary_call(op, st[r][sp], tos[r]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = rc_unary_call(op, tos[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) res[r] = (sp > 0) ? st[r][0] : tos[r];
+}
+
+/* Map workgroup g to (program, tile) so that XCD g % 8 owns a contiguous
+   eighth of the tiles. Returns false for a padding workgroup. */
+__device__ __forceinline__ bool rc_item_of(int g, int nprog, int tiles, int *p, int *t)
+{
+    if (tiles < 8) {
+        if (g >= nprog * tiles) return false;
+        *p = g / tiles;
+        *t = g % tiles;
+        return true;
+    }
+    const int x = g & 7;
+    const int j = g >> 3;
+    const int t0 = (int)(((long long)tiles * x) >> 3);
+    const int t1 = (int)(((long long)tiles * (x + 1)) >> 3);
+    const int tt = j / nprog;
+    if (tt >= t1 - t0) return false;
+    *t = t0 + tt;
+    *p = j - tt * nprog;
+    return true;
+}
+
+template <int S, int R, int MODE>
+__global__ __launch_bounds__(64) void rc_eval_kernel(
+    const float *__restrict__ cols, const float *__restrict__ y, int n,
+    const uint32_t *__restrict__ code, const uint32_t *__restrict__ meta,
+    const uint32_t *__restrict__ plist, int ncls, int tiles, int tile_rows, int kind,
+    double *__restrict__ part, int *__restrict__ part_bad, float *__restrict__ pred_out)
+{
+    int j, t;
+    if (!rc_item_of(blockIdx.x, ncls, tiles, &j, &t)) return;
+    const int p = (int)plist[j];
+    const int lane = threadIdx.x;
+    const uint32_t *pc = code + meta[p];
+    const int row0 = t * tile_rows;
+    const int row1 = min(n, row0 + tile_rows);
+    double acc = 0.0;
+    int bad = 0;
+    for (int base = row0; base < row1; base += 64 * R) {
+        int xrow[R];
+        bool ok[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = base + r * 64 + lane;
+            ok[r] = row < row1;
+            xrow[r] = ok[r] ? row : row0;
+        }
+        float res[R];
+        rc_run<S, R>(pc, cols, n, xrow, res);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!ok[r]) continue;
+            const float pred = res[r];
+            if (MODE == MODE_PREDICT) {
+                pred_out[(size_t)p * (size_t)n + (size_t)xrow[r]] = pred;
+                continue;
+            }
+            const bool hard = !isfinite(pred) || fabsf(pred) > 1e30f;
+            if (MODE == MODE_EVAL) {
+                if (hard) {
+                    acc += 1e30;
+                } else {
+                    /* rc_row_loss (eval.cu:648-664) on the f32 residual. */
+                    const float rr = pred - y[xrow[r]];
+                    float l;
+                    if (kind == RC_LOSS_LOG_SQUASHED) {
+                        const float q = rr * rr;
+                        l = isfinite(q) ? log1pf(q) : 2.0f * logf(fabsf(rr));
+                    } else if (kind == RC_LOSS_ABSOLUTE) {
+                        l = fabsf(rr);
+                    } else if (kind == RC_LOSS_SQUARED) {
+                        l = rr * rr;
+                    } else {
+                        l = RC_F32_NAN;
+                    }
+                    acc += (double)l;
+                }
+            } else {
+                if (hard) {
+                    bad = 1;
+                    continue;
+                }
+                const double rr = (double)pred - (double)y[xrow[r]];
+                if (kind == RC_SCORE_ABSOLUTE) {
+                    acc += fabs(rr);
+                } else if (kind == RC_SCORE_LOG_SQUASHED) {
+                    const double q = rr * rr;
+                    acc += isfinite(q) ? log1p(q) : 2.0 * log(fabs(rr));
+                } else if (kind == RC_SCORE_SIGNED) {
+                    acc += rr;
+                } else if (kind == RC_SCORE_MAXIMUM) {
+                    acc = fmax(acc, fabs(rr));
+                } else {
+                    acc += rr * rr;
+                }
+            }
+        }
+    }
+    if (MODE == MODE_PREDICT) return;
+    double tot;
+    if (MODE == MODE_SCORE && kind == RC_SCORE_MAXIMUM) {
+        tot = acc;
+        for (int off = 32; off > 0; off >>= 1) tot = fmax(tot, __shfl_xor(tot, off, 64));
+    } else {
+        tot = rc_wave_sum_f64(acc);
+    }
+    const int anybad = __any(bad) ? 1 : 0;
+    if (lane == 0) {
+        const size_t item = (size_t)p * (size_t)tiles + (size_t)t;
+        part[item] = tot;
+        part_bad[item] = anybad;
+    }
+}
+
+/* Per-program finish in tile order (deterministic). */
+__global__ void rc_eval_finish(const double *__restrict__ part, const int *__restrict__ part_bad,
+                               int nprog, int tiles, int n, int mode, int kind,
+                               float *__restrict__ out32, double *__restrict__ out64,
+                               int *__restrict__ status)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprog) return;
+    double s = 0.0;
+    int bad = 0;
+    for (int t = 0; t < tiles; ++t) {
+        const size_t item = (size_t)p * (size_t)tiles + (size_t)t;
+        if (mode == MODE_SCORE && kind == RC_SCORE_MAXIMUM) s = fmax(s, part[item]);
+        else s += part[item];
+        bad |= part_bad[item];
+    }
+    if (mode == MODE_EVAL) {
+        out32[p] = (float)s;
+        return;
+    }
+    if (bad) {
+        out64[p] = __builtin_inf();
+        status[p] = RC_SCORE_NONFINITE;
+        return;
+    }
+    status[p] = 0;
+    if (kind == RC_SCORE_AIC || kind == RC_SCORE_AICC || kind == RC_SCORE_BIC) out64[p] = s;
+    else if (kind == RC_SCORE_SIGNED) out64[p] = fabs(s) / (double)n;
+    else if (kind == RC_SCORE_MAXIMUM) out64[p] = s;
+    else out64[p] = s / (double)n;
+}
+
+/* FNV-1a over a fixed-width slot up to and including HALT (eval.cu:2665-2681). */
+__global__ void rc_hash_kernel(const uint32_t *__restrict__ prog, int width, int nprog,
+                               unsigned long long *__restrict__ out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprog) return;
+    const uint32_t *slot = prog + (size_t)p * (size_t)width;
+    unsigned long long h = 0xcbf29ce484222325ULL;
+    for (int q = 0; q < width; ++q) {
+        const uint32_t w = slot[q];
+        h ^= (unsigned long long)w;
+        h *= 0x00000100000001b3ULL;
+        if (w == OP_HALT) break;
+    }
+    out[p] = h;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Host side.                                                                */
+
+namespace {
+
+/* Row tiles are a multiple of 256 rows so every class (R = 4, 2, 1) walks
+   whole passes; tiles depend only on n, so a program's sums never depend on
+   the batch it came in. */
+int tile_rows_for(int n)
+{
+    if (n <= 4096) return ((n + 255) / 256) * 256 > 0 ? ((n + 255) / 256) * 256 : 256;
+    return 4096;
+}
+
+template <int S, int R>
+void launch_class(int mode, dim3 grid, hipStream_t s, const float *cols, const float *y, int n,
+                  const uint32_t *code, const uint32_t *meta, const uint32_t *plist, int ncls,
+                  int tiles, int tile_rows, int kind, double *part, int *part_bad, float *pred)
+{
+    if (mode == MODE_EVAL)
+        rc_eval_kernel<S, R, MODE_EVAL><<<grid, 64, 0, s>>>(cols, y, n, code, meta, plist, ncls, tiles,
+                                                            tile_rows, kind, part, part_bad, pred);
+    else if (mode == MODE_SCORE)
+        rc_eval_kernel<S, R, MODE_SCORE><<<grid, 64, 0, s>>>(cols, y, n, code, meta, plist, ncls, tiles,
+                                                             tile_rows, kind, part, part_bad, pred);
+    else
+        rc_eval_kernel<S, R, MODE_PREDICT><<<grid, 64, 0, s>>>(cols, y, n, code, meta, plist, ncls,
+                                                               tiles, tile_rows, kind, part, part_bad,
+                                                               pred);
+}
+
+int launch_rows(RcCtx *c, int mode, int kind, int nprog, const uint32_t *code,
+                const uint32_t *meta, hipStream_t s, float *pred)
+{
+    RcRows *r = &c->sets[0];
+    const int n = r->n;
+    if (n <= 0) return -1;
+    const int tile_rows = tile_rows_for(n);
+    const int tiles = (n + tile_rows - 1) / tile_rows;
+    const size_t items = (size_t)nprog * (size_t)tiles;
+    if (rc_grow(c, (void **)&c->d_part, &c->part_cap, items, sizeof(double)) != 0) return -1;
+    if (rc_grow(c, (void **)&c->d_part_bad, &c->part_bad_cap, items, sizeof(int)) != 0) return -1;
+    for (int cls = 0; cls < 4; ++cls) {
+        const int ncls = c->cls_start[cls + 1] - c->cls_start[cls];
+        if (ncls <= 0) continue;
+        const uint32_t *plist = meta + nprog + c->cls_start[cls];
+        size_t grid;
+        if (tiles < 8) grid = (size_t)ncls * (size_t)tiles;
+        else grid = ((size_t)(tiles + 7) / 8) * (size_t)ncls * 8;
+        if (grid > 0x7fffffffu) return -1;
+        dim3 g((unsigned)grid);
+        if (cls == 0)
+            launch_class<8, 4>(mode, g, s, r->cols, r->y, n, code, meta, plist, ncls, tiles, tile_rows,
+                               kind, c->d_part, c->d_part_bad, pred);
+        else if (cls == 1)
+            launch_class<16, 2>(mode, g, s, r->cols, r->y, n, code, meta, plist, ncls, tiles, tile_rows,
+                                kind, c->d_part, c->d_part_bad, pred);
+        else if (cls == 2)
+            launch_class<32, 1>(mode, g, s, r->cols, r->y, n, code, meta, plist, ncls, tiles, tile_rows,
+                                kind, c->d_part, c->d_part_bad, pred);
+        else
+            launch_class<64, 1>(mode, g, s, r->cols, r->y, n, code, meta, plist, ncls, tiles, tile_rows,
+                                kind, c->d_part, c->d_part_bad, pred);
+        if (hipGetLastError() != hipSuccess) return -1;
+    }
+    if (mode == MODE_PREDICT) return 0;
+    rc_eval_finish<<<(nprog + 127) / 128, 128, 0, s>>>(c->d_part, c->d_part_bad, nprog, tiles, n,
+                                                       mode, kind, c->d_out, c->d_score,
+                                                       c->d_status);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+/* Upload a translated batch into the next arena slot. */
+int upload_translated(RcCtx *c, const std::vector<uint32_t> &code, const std::vector<uint32_t> &meta,
+                      int nprog)
+{
+    RcStats *st = &c->stats;
+    double t = rc_now_ms();
+    const int slot = c->arena_slot ^ 1;
+    /* Capacity is per slot; a growth reallocates both slots. */
+    if (code.size() > c->arena_code_words || meta.size() > c->arena_meta_words ||
+        !c->arena_code[slot]) {
+        (void)hipStreamSynchronize(c->s_eval);
+        size_t wc = code.size() + code.size() / 4 + 256;
+        size_t wm = meta.size() + meta.size() / 4 + 64;
+        for (int i = 0; i < 2; ++i) {
+            if (c->arena_code[i]) rc_dev_free(c, c->arena_code[i]);
+            if (c->arena_meta[i]) rc_dev_free(c, c->arena_meta[i]);
+            c->arena_code[i] = nullptr;
+            c->arena_meta[i] = nullptr;
+            if (rc_dev_alloc(c, (void **)&c->arena_code[i], wc * 4) != 0) return -1;
+            if (rc_dev_alloc(c, (void **)&c->arena_meta[i], wm * 4) != 0) return -1;
+        }
+        c->arena_code_words = wc;
+        c->arena_meta_words = wm;
+    }
+    const size_t nb = (code.size() + meta.size()) * 4;
+    if (c->async_copy && rc_grow_pinned(c, (void **)&c->h_stage, &c->h_stage_bytes, nb) != 0)
+        c->async_copy = 0;
+    st->alloc_ms += rc_now_ms() - t;
+    t = rc_now_ms();
+    if (c->async_copy) {
+        /* The staging buffer may still feed the previous copy. */
+        (void)hipStreamSynchronize(c->s_eval);
+        memcpy(c->h_stage, code.data(), code.size() * 4);
+        memcpy(c->h_stage + code.size(), meta.data(), meta.size() * 4);
+        if (hipMemcpyAsync(c->arena_code[slot], c->h_stage, code.size() * 4, hipMemcpyHostToDevice,
+                           c->s_eval) != hipSuccess)
+            return -1;
+        if (hipMemcpyAsync(c->arena_meta[slot], c->h_stage + code.size(), meta.size() * 4,
+                           hipMemcpyHostToDevice, c->s_eval) != hipSuccess)
+            return -1;
+    } else {
+        if (hipMemcpy(c->arena_code[slot], code.data(), code.size() * 4, hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(c->arena_meta[slot], meta.data(), meta.size() * 4, hipMemcpyHostToDevice) !=
+                hipSuccess)
+            return -1;
+    }
+    st->h2d_ms += rc_now_ms() - t;
+    st->h2d_bytes += (unsigned long long)nb;
+    c->arena_slot = slot;
+    c->nprog = nprog;
+    st->batch = nprog;
+    return 0;
+}
+
+int ensure_outputs(RcCtx *c, int nprog)
+{
+    if (rc_grow(c, (void **)&c->d_out, &c->out_cap, (size_t)nprog, sizeof(float)) != 0) return -1;
+    size_t cap = c->score_cap;
+    if (rc_grow(c, (void **)&c->d_score, &cap, (size_t)nprog, sizeof(double)) != 0) return -1;
+    size_t cap2 = c->score_cap;
+    if (rc_grow(c, (void **)&c->d_status, &cap2, (size_t)nprog, sizeof(int)) != 0) return -1;
+    c->score_cap = cap < cap2 ? cap : cap2;
+    return 0;
+}
+
+float score_current(RcCtx *c, int nprog, int kind, const int *pcount, double *out, int *status)
+{
+    RcStats *st = &c->stats;
+    if (!(kind >= 0 && kind <= 4) && !(kind >= 7 && kind <= 9)) {
+        st->failed = 1;
+        return -1.0f;
+    }
+    if (ensure_outputs(c, nprog) != 0) return -1.0f;
+    double t = rc_now_ms();
+    (void)hipEventRecord(c->t0, c->s_eval);
+    if (launch_rows(c, MODE_SCORE, kind, nprog, c->arena_code[c->arena_slot],
+                    c->arena_meta[c->arena_slot], c->s_eval, nullptr) != 0) {
+        st->failed = 1;
+        return -1.0f;
+    }
+    (void)hipEventRecord(c->t1, c->s_eval);
+    st->launch_ms += rc_now_ms() - t;
+    t = rc_now_ms();
+    if (hipEventSynchronize(c->t1) != hipSuccess || hipGetLastError() != hipSuccess) {
+        st->failed = 1;
+        return -1.0f;
+    }
+    st->sync_ms += rc_now_ms() - t;
+    t = rc_now_ms();
+    if (hipMemcpy(out, c->d_score, sizeof(double) * (size_t)nprog, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(status, c->d_status, sizeof(int) * (size_t)nprog, hipMemcpyDeviceToHost) != hipSuccess) {
+        st->failed = 1;
+        return -1.0f;
+    }
+    st->d2h_ms += rc_now_ms() - t;
+    st->d2h_bytes += (unsigned long long)nprog * 12ull;
+    const int n = c->sets[0].n;
+    if (kind == RC_SCORE_AIC || kind == RC_SCORE_AICC || kind == RC_SCORE_BIC) {
+        /* The AIC family's parameter penalty, applied on the host so k may
+           differ per candidate (eval.cu:3632-3645, metrics.rs). */
+        for (int i = 0; i < nprog; ++i) {
+            if (status[i] != 0) continue;
+            const double dn = (double)n;
+            const double dk = pcount ? (double)pcount[i] : 0.0;
+            const double rss = out[i];
+            const double base = (rss <= 0.0) ? -__builtin_inf() : dn * log(rss / dn);
+            if (kind == RC_SCORE_AIC) {
+                out[i] = base + 2.0 * dk;
+            } else if (kind == RC_SCORE_BIC) {
+                out[i] = base + dk * log(dn);
+            } else if (!std::isfinite(base)) {
+                out[i] = base;
+            } else {
+                const double denom = dn - dk - 1.0;
+                out[i] = denom <= 0.0 ? __builtin_inf()
+                                      : base + 2.0 * dk + 2.0 * dk * (dk + 1.0) / denom;
+            }
+        }
+    }
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, c->t0, c->t1);
+    st->kernel_ms = ms;
+    st->batch = nprog;
+    st->wall_ms = rc_now_ms() - c->batch_start_ms;
+    c->warm = 1;
+    c->batches++;
+    return ms;
+}
+
+} /* namespace */
+
+extern "C" {
+
+int rc_set_programs(void *ctx, const unsigned int *prog, const unsigned int *off, int nprog)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || !prog || !off || nprog <= 0) return -1;
+    if (c->pending) {
+        /* The reference refuses arena growth while a batch is in flight; the
+           staging here is synchronised instead, and the overlap is counted. */
+        c->overlapped_uploads++;
+    }
+    std::vector<uint32_t> code, meta;
+    c->max_stack = rc_translate_programs(prog, off, nprog, c->sets[0].v, code, meta, c->cls_start);
+    if (upload_translated(c, code, meta, nprog) != 0) {
+        c->stats.failed = 1;
+        return -1;
+    }
+    c->stats.wall_ms = rc_now_ms() - c->batch_start_ms;
+    return 0;
+}
+
+int rc_eval_submit(void *ctx, int loss)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || c->nprog <= 0 || c->pending) return -1;
+    if (ensure_outputs(c, c->nprog) != 0) return -1;
+    double t = rc_now_ms();
+    (void)hipEventRecord(c->t0, c->s_eval);
+    if (launch_rows(c, MODE_EVAL, loss, c->nprog, c->arena_code[c->arena_slot],
+                    c->arena_meta[c->arena_slot], c->s_eval, nullptr) != 0) {
+        c->stats.failed = 1;
+        return -1;
+    }
+    (void)hipEventRecord(c->t1, c->s_eval);
+    c->stats.launch_ms += rc_now_ms() - t;
+    c->pending = 1;
+    c->pending_nprog = c->nprog;
+    return 0;
+}
+
+float rc_eval_collect(void *ctx, float *out)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || !out || !c->pending) return -1.0f;
+    RcStats *st = &c->stats;
+    double t = rc_now_ms();
+    const hipError_t e = hipEventSynchronize(c->t1);
+    st->sync_ms += rc_now_ms() - t;
+    c->pending = 0;
+    if (e != hipSuccess || hipGetLastError() != hipSuccess) {
+        st->failed = 1;
+        return -1.0f;
+    }
+    t = rc_now_ms();
+    if (hipMemcpy(out, c->d_out, sizeof(float) * (size_t)c->pending_nprog, hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+        st->failed = 1;
+        return -1.0f;
+    }
+    st->d2h_ms += rc_now_ms() - t;
+    st->d2h_bytes += (unsigned long long)c->pending_nprog * 4ull;
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, c->t0, c->t1);
+    st->kernel_ms = ms;
+    st->batch = c->pending_nprog;
+    st->wall_ms = rc_now_ms() - c->batch_start_ms;
+    c->warm = 1;
+    c->batches++;
+    return ms;
+}
+
+float rc_eval(void *ctx, float *out, int loss)
+{
+    if (rc_eval_submit(ctx, loss) != 0) return -1.0f;
+    return rc_eval_collect(ctx, out);
+}
+
+int rc_predict(void *ctx, const unsigned int *prog, const unsigned int *off, int nprog, float *out)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || nprog <= 0 || !out) return -1;
+    if (rc_set_programs(ctx, prog, off, nprog) != 0) return -1;
+    const size_t total = (size_t)nprog * (size_t)c->sets[0].n;
+    float *d_pred = nullptr;
+    if (hipMalloc((void **)&d_pred, sizeof(float) * (total ? total : 1)) != hipSuccess) return -1;
+    int rc = launch_rows(c, MODE_PREDICT, 0, nprog, c->arena_code[c->arena_slot],
+                         c->arena_meta[c->arena_slot], c->s_eval, d_pred);
+    if (rc == 0 && hipStreamSynchronize(c->s_eval) != hipSuccess) rc = -1;
+    if (rc == 0 && total &&
+        hipMemcpy(out, d_pred, sizeof(float) * total, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = -1;
+    (void)hipFree(d_pred);
+    return rc;
+}
+
+float rc_score_batch(void *ctx, const unsigned int *prog, const unsigned int *off, int nprog, int kind,
+                     const int *pcount, double *out, int *status)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || nprog <= 0 || !out || !status) return -1.0f;
+    if (rc_set_programs(ctx, prog, off, nprog) != 0) return -1.0f;
+    return score_current(c, nprog, kind, pcount, out, status);
+}
+
+int rc_pool_reserve(void *ctx, int nprog, int width)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || nprog <= 0 || width <= 0) return -1;
+    if (c->pool_cap >= nprog && c->pool_width == width) return 0;
+    for (int i = 0; i < 2; ++i) {
+        if (c->pool_raw[i]) rc_dev_free(c, c->pool_raw[i]);
+        c->pool_raw[i] = nullptr;
+        if (rc_dev_alloc(c, (void **)&c->pool_raw[i], sizeof(uint32_t) * (size_t)nprog * width) != 0)
+            return -1;
+        c->pool_host[i].assign((size_t)nprog * width, OP_HALT);
+    }
+    if (c->d_hash) rc_dev_free(c, c->d_hash);
+    c->d_hash = nullptr;
+    if (rc_dev_alloc(c, (void **)&c->d_hash, sizeof(unsigned long long) * (size_t)nprog) != 0) return -1;
+    c->pool_cap = nprog;
+    c->pool_width = width;
+    return 0;
+}
+
+int rc_pool_upload(void *ctx, int which, const unsigned int *prog, int nprog)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || !prog || nprog <= 0 || nprog > c->pool_cap || !c->pool_raw[which & 1]) return -1;
+    const size_t words = (size_t)nprog * (size_t)c->pool_width;
+    double t = rc_now_ms();
+    memcpy(c->pool_host[which & 1].data(), prog, words * 4);
+    if (hipMemcpy(c->pool_raw[which & 1], prog, words * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    c->stats.h2d_ms += rc_now_ms() - t;
+    c->stats.h2d_bytes += words * 4;
+    c->stats.batch = nprog;
+    return 0;
+}
+
+static int pool_translate(RcCtx *c, int which, int nprog, int width)
+{
+    if (!c || nprog <= 0 || nprog > c->pool_cap || width != c->pool_width) return -1;
+    std::vector<uint32_t> off((size_t)nprog + 1);
+    for (int p = 0; p <= nprog; ++p) off[p] = (uint32_t)((size_t)p * width);
+    std::vector<uint32_t> code, meta;
+    c->max_stack = rc_translate_programs(c->pool_host[which & 1].data(), off.data(), nprog,
+                                         c->sets[0].v, code, meta, c->cls_start);
+    return upload_translated(c, code, meta, nprog);
+}
+
+float rc_pool_eval(void *ctx, int which, int nprog, int width, float *out, int loss)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || !out || pool_translate(c, which, nprog, width) != 0) return -1.0f;
+    return rc_eval(ctx, out, loss);
+}
+
+float rc_pool_score(void *ctx, int which, int nprog, int width, int kind, const int *pcount,
+                    double *out, int *status)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || !out || !status || pool_translate(c, which, nprog, width) != 0) return -1.0f;
+    return score_current(c, nprog, kind, pcount, out, status);
+}
+
+int rc_pool_hash(void *ctx, int which, int nprog, int width, unsigned long long *out)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || !out || !c->pool_raw[which & 1] || !c->d_hash || nprog <= 0 || nprog > c->pool_cap ||
+        width != c->pool_width)
+        return -1;
+    rc_hash_kernel<<<(nprog + 127) / 128, 128, 0, c->s_eval>>>(c->pool_raw[which & 1], width, nprog,
+                                                               c->d_hash);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->s_eval) != hipSuccess) return -1;
+    if (hipMemcpy(out, c->d_hash, sizeof(unsigned long long) * (size_t)nprog, hipMemcpyDeviceToHost) !=
+        hipSuccess)
+        return -1;
+    return 0;
+}
+
+} /* extern "C" */

@@ -1,0 +1,1494 @@
+/*
+ * Batched Levenberg-Marquardt / IRLS constant fitting on gfx950.
+ *
+ * Reference behaviour: fit_kernel (eval.cu:2265-2606) with ft_forward /
+ * ft_backward (1173-1317), ft_assemble (1693-1804), ft_loss (1809-1855),
+ * ft_solve (1874-1949), the f64 re-baseline and Gauss-Newton polish
+ * (2536-2580), the status labels (2583-2605) and rc_fit_batch (4001-4078).
+ *
+ * MI355X design.  The reference runs one thread block per candidate for the
+ * whole fit, so a candidate's rows are limited to one SM and uneven candidates
+ * leave SMs idle.  Here every LM iteration is a short sequence of batch-wide
+ * launches over work items (candidate, 8192-row tile), so all 256 CUs share
+ * every pass whatever the mix of program lengths, constant counts and row
+ * counts:
+ *   ft_asm_kernel    f32 forward + adjoint per row (one wave per item), the
+ *                    augmented row [s*J | s*r_hi | s*r_lo] written to LDS and
+ *                    reduced on FP32 MFMA (v_mfma_f32_16x16x4f32): J^T J,
+ *                    J^T r and r^T r in one product.  r = pred - y is formed in
+ *                    f64 and split hi/lo so J^T r keeps f64-level accuracy;
+ *                    the MFMA accumulators are promoted to f64 every 256 rows;
+ *   ft_trial_kernel  forward only: sum w*rho(r) in f64 (f32 rows in the LM
+ *                    loop, f64 rows for the exact re-baseline and the polish);
+ *   ft_ctrl_kernel   one wave per candidate: fixed-order reduction of the
+ *                    item partials, the FP64 damped solve, acceptance, lambda
+ *                    schedule, IRLS rounds and status labels.
+ * Partials are per (candidate, tile) and reduced in tile order, so a
+ * candidate's result depends only on its own program and rows, never on the
+ * batch it came in.  The launch sequence is recorded once per batch shape as
+ * a hipGraph and replayed.
+ */
+#include "rc_fit_dev.h"
+#include "rc_internal.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+namespace {
+
+constexpr int XS = 66;          /* LDS column stride of the augmented rows */
+constexpr int TILE_ROWS = 8192; /* rows per work item (multiple of 256) */
+/* Interpreter capacities.  Every vector update is unconditional (the op's
+   arithmetic runs on scalars), so a slot above the live area is written each
+   step: the small class needs depth + 1 value slots, adjoint depth + 2 slots
+   and 31 entries per tape lane. */
+constexpr int SMALL_S = 8, SMALL_AS = 8, SMALL_TAPE = 32;
+constexpr int BIG_S = 100, BIG_AS = 100, BIG_TAPE = 100;
+
+/* Candidate stages of the device state machine. */
+enum {
+    ST_NEED_ASM = 1, ST_NEED_TRIAL = 2, ST_LM_DONE = 3, ST_P_NEED_ASM = 4, ST_P_NEED_TRIAL = 5,
+    ST_DONE = 6
+};
+
+/* Control operations. */
+enum {
+    CT_INIT = 0, CT_AFTER_ASM = 1, CT_AFTER_TRIAL = 2, CT_PRE_REBASE = 3, CT_REBASE = 4,
+    CT_AFTER_ASM_P = 5, CT_AFTER_TRIAL_P = 6, CT_FINAL = 7
+};
+
+struct FtState {
+    double c[FT_KMAX];
+    double trial[FT_KMAX];
+    double jtj[FT_KMAX * (FT_KMAX + 1) / 2];
+    double jtr[FT_KMAX];
+    double best;
+    double lambda;
+    int k, metric, irls, rounds, inner, iters;
+    int stage, it, round, assembled, first;
+    int status, sstart, srank, improved, skipped;
+    int halvings, polish_it, pre_status, pad;
+    int n_asm, n_trial, n_exact, pad2; /* passes consumed (telemetry) */
+};
+
+/* Wave-uniform storage for the interpreter: an LLVM vector (registers,
+   indexed through s_set_gpr_idx) for the common sizes, a plain array for the
+   rare oversized program.  Callers update them unconditionally (one indexed
+   read and one indexed write per step) so no branch join ever merges two
+   versions of a vector. */
+template <typename T, int N> struct VStore;
+template <> struct VStore<float, 8> { fv8 v; __device__ float get(int i) const { return v[i]; } __device__ void set(int i, float x) { v[i] = x; } };
+template <> struct VStore<float, 32> { fv32 v; __device__ float get(int i) const { return v[i]; } __device__ void set(int i, float x) { v[i] = x; } };
+template <> struct VStore<double, 8> { dv8 v; __device__ double get(int i) const { return v[i]; } __device__ void set(int i, double x) { v[i] = x; } };
+template <typename T, int N> struct AStore {
+    T a[N];
+    __device__ T get(int i) const { return a[i]; }
+    __device__ void set(int i, T x) { a[i] = x; }
+};
+
+template <typename T, bool BIG> struct Stores;
+template <> struct Stores<float, false> {
+    typedef VStore<float, SMALL_S> Val;
+    typedef VStore<float, SMALL_AS> Adj;
+    typedef VStore<float, SMALL_TAPE> Tape;
+};
+template <> struct Stores<float, true> {
+    typedef AStore<float, BIG_S> Val;
+    typedef AStore<float, BIG_AS> Adj;
+    typedef AStore<float, BIG_TAPE> Tape;
+};
+template <> struct Stores<double, false> {
+    typedef VStore<double, SMALL_S> Val;
+    typedef VStore<double, SMALL_AS> Adj;
+    typedef AStore<double, 1> Tape;
+};
+template <> struct Stores<double, true> {
+    typedef AStore<double, BIG_S> Val;
+    typedef AStore<double, BIG_AS> Adj;
+    typedef AStore<double, 1> Tape;
+};
+
+template <typename T> struct XVec;
+template <> struct XVec<float> { typedef fv8 type; };
+template <> struct XVec<double> { typedef dv8 type; };
+
+/* The two LIFO lanes of the tape: partials w.r.t. a left (or only) child and
+   w.r.t. a right child.  Each lane's pushes and pops mirror each other. */
+template <typename T, bool BIG> struct Tape2 {
+    typename Stores<T, BIG>::Tape l, r;
+    int tl = 0, tr = 0;
+};
+
+/* Forward sweep of one row (stack machine, top of stack cached).  With TAPE,
+   on-path nodes push their local partials (FF_D0 / FF_D1). */
+template <typename T, bool BIG, bool TAPE>
+__device__ __forceinline__ T ft_forward(const uint32_t *__restrict__ fc, int m,
+                                        const T *__restrict__ cst,
+                                        const typename XVec<T>::type &xv,
+                                        const T *__restrict__ cols, int n, int row,
+                                        Tape2<T, BIG> &tape)
+{
+    typename Stores<T, BIG>::Val st;
+    T tos = (T)0;
+    int sp = 0; /* st[1 .. sp-1] hold the values below the top; st[0] is scratch */
+    for (int i = 0; i < m; ++i) {
+        const uint32_t w = fc[i];
+        const uint32_t op = w & 0xffu;
+        const uint32_t fl = (w >> 8) & 0xffu;
+        const uint32_t arg = w >> 16;
+        const T a = st.get(sp > 0 ? sp - 1 : 0);
+        const T b = tos;
+        T res, d0 = (T)0, d1 = (T)0;
+        int push = 0, pop = 0;
+        if (op == OP_CONST) {
+            res = cst[arg];
+            push = 1;
+        } else if (op == OP_VAR) {
+            res = (arg < 8u) ? xv[arg] : cols[(size_t)arg * (size_t)n + (size_t)row];
+            push = 1;
+        } else if (op == OP_MUL) {
+            res = a * b;
+            d0 = b;
+            d1 = a;
+            pop = 1;
+        } else if (op == OP_ADD) {
+            res = a + b;
+            pop = 1;
+        } else if (op == OP_SUB) {
+            res = a - b;
+            pop = 1;
+        } else if (op == OP_DIV || op == OP_POW) {
+            res = ft_binary(op, a, b);
+            if (TAPE && fl) {
+                if (op == OP_DIV) { d0 = (T)1 / b; d1 = -a / (b * b); }
+                else if (a > (T)0) { d0 = b * pow(a, b - (T)1); d1 = pow(a, b) * log(a); }
+            }
+            pop = 1;
+        } else {
+            res = ft_unary(op, b);
+            if (TAPE && fl) d0 = ft_unary_partial<T>(op, b, res);
+        }
+        st.set(sp, b);          /* the old top: kept only on a push */
+        sp += push - pop;
+        tos = res;
+        if (TAPE) {
+            tape.l.set(tape.tl, d0);
+            tape.r.set(tape.tr, d1);
+            tape.tl += (fl & FF_D0) ? 1 : 0;
+            tape.tr += (fl & FF_D1) ? 1 : 0;
+        }
+    }
+    return tos;
+}
+
+/* Reverse sweep seeded with `seed` at the root.  Constant gradients go to
+   the LDS column block X (X[slot * XS + lane]); input gradients, when asked,
+   to gx.  Returns whether every constant gradient is finite. */
+template <typename T, bool BIG>
+__device__ __forceinline__ bool ft_reverse(const uint32_t *__restrict__ rc, int nrev, T seed,
+                                           Tape2<T, BIG> &tape, float *X, int lane, T *gx)
+{
+    typename Stores<T, BIG>::Adj ad;
+    ad.set(0, seed);
+    int ap = 1;
+    bool fin = true;
+    for (int j = 0; j < nrev; ++j) {
+        const uint32_t w = rc[j];
+        const uint32_t kind = w & 0xffu;
+        const uint32_t fl = (w >> 8) & 0xffu;
+        const uint32_t arg = w >> 16;
+        const T a = ad.get(ap - 1);
+        const T dl = tape.l.get(tape.tl > 0 ? tape.tl - 1 : 0);
+        const T dr = tape.r.get(tape.tr > 0 ? tape.tr - 1 : 0);
+        const bool z = (a == (T)0);
+        T first = (T)0, second = (T)0;
+        int npush = 0;
+        if (kind == RK_CONST) {
+            if (X) X[arg * XS + lane] = (float)a;
+            fin = fin && isfinite(a);
+        } else if (kind == RK_VAR) {
+            if (gx) gx[arg] += a;
+        } else if (kind == RK_ADD || kind == RK_SUB) {
+            const T ar = (kind == RK_SUB) ? -a : a;
+            if ((fl & RF_LEFT) && (fl & RF_RIGHT)) { first = a; second = ar; npush = 2; }
+            else if (fl & RF_LEFT) { first = a; npush = 1; }
+            else { first = ar; npush = 1; }
+        } else if (kind == RK_NEG) {
+            first = -a;
+            npush = 1;
+        } else if (kind == RK_UNARY) {
+            first = z ? (T)0 : a * dl;
+            npush = 1;
+        } else {
+            const T gl = z ? (T)0 : a * dl;
+            const T gr = z ? (T)0 : a * dr;
+            if ((fl & RF_LEFT) && (fl & RF_RIGHT)) { first = gl; second = gr; npush = 2; }
+            else if (fl & RF_LEFT) { first = gl; npush = 1; }
+            else { first = gr; npush = 1; }
+        }
+        ad.set(ap - 1, first);
+        ad.set(ap, second);
+        ap += npush - 1;
+        tape.tl -= (kind == RK_UNARY || (kind == RK_BINARY && (fl & RF_LEFT))) ? 1 : 0;
+        tape.tr -= (kind == RK_BINARY && (fl & RF_RIGHT)) ? 1 : 0;
+    }
+    return fin;
+}
+
+__device__ __forceinline__ void load_x(const float *cols, int n, int v, int row, fv8 &xv)
+{
+#pragma unroll
+    for (int q = 0; q < 8; ++q) xv[q] = (q < v) ? cols[(size_t)q * (size_t)n + (size_t)row] : 0.0f;
+}
+
+__device__ __forceinline__ void load_x(const double *cols, int n, int v, int row, dv8 &xv)
+{
+#pragma unroll
+    for (int q = 0; q < 8; ++q) xv[q] = (q < v) ? cols[(size_t)q * (size_t)n + (size_t)row] : 0.0;
+}
+
+/* ---- assembly: forward + adjoint + MFMA Gram ------------------------------ */
+
+/* POLISH: the exact polish's assembly.  The residual comes from an f64
+   forward pass on the f64 rows (the Jacobian stays f32), so the iteration's
+   fixed point is the f64 optimum; a non-finite gradient row is skipped and
+   counted instead of failing the candidate (ft_assemble_d semantics). */
+template <int KC, bool BIG, bool POLISH>
+__global__ __launch_bounds__(64) void ft_asm_kernel(
+    const FtItem *__restrict__ items, const int *__restrict__ list, int nlist,
+    const FtCand *__restrict__ cands, const FtRows *__restrict__ rowsd,
+    const uint32_t *__restrict__ code, const int *__restrict__ run, const float *__restrict__ cp32,
+    const double *__restrict__ cp64, double *__restrict__ pgram, int tri_stride,
+    double *__restrict__ ploss, int *__restrict__ pbad, int *__restrict__ pskip)
+{
+    const bool polish = POLISH;
+    const int li = blockIdx.x;
+    if (li >= nlist) return;
+    const int item = list[li];
+    const FtItem it = items[item];
+    const int cand = it.cand;
+    if (!run[cand]) return;
+    const FtCand d = cands[cand];
+    const FtRows R = rowsd[d.rowset];
+    const int lane = threadIdx.x;
+    const int k = d.k;
+
+    __shared__ float X[KC * XS];
+    for (int i = lane; i < KC * XS; i += 64) X[i] = 0.0f;
+    __syncthreads();
+
+    const float *cst = cp32 + (size_t)cand * FT_KMAX;
+    const uint32_t *fc = code + d.fwd_off;
+    const uint32_t *rc = code + d.rev_off;
+    floatx4 acc00 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
+    double g00[4] = {0, 0, 0, 0}, g10[4] = {0, 0, 0, 0}, g11[4] = {0, 0, 0, 0};
+    double lsum = 0.0;
+    int bad = 0, skip = 0;
+    const int row_end = it.row0 + it.rows;
+    int step = 0;
+    for (int base = it.row0; base < row_end; base += 64, ++step) {
+        const int row = base + lane;
+        bool live = row < row_end;
+        const int rr = live ? row : it.row0;
+        const double wb = R.w64 ? R.w64[rr] : 1.0;
+        live = live && (wb > 0.0);
+        fv8 xv;
+        load_x(R.cols, R.n, R.v, rr, xv);
+        Tape2<float, BIG> tape;
+        const float pred = ft_forward<float, BIG, true>(fc, d.m, cst, xv, R.cols, R.n, rr, tape);
+        double p64 = (double)pred;
+        if (POLISH) {
+            dv8 xd;
+            load_x(R.cols64, R.n, R.v, rr, xd);
+            Tape2<double, BIG> none;
+            p64 = ft_forward<double, BIG, false>(fc, d.m, cp64 + (size_t)cand * FT_KMAX, xd,
+                                                 R.cols64, R.n, rr, none);
+        }
+        const double r64 = p64 - R.y64[rr];
+        const bool ok = live && isfinite(p64) && (polish || fabsf(pred) <= 1e30f) && isfinite(r64);
+        const double s64 = sqrt(rc_row_weight(wb, r64, d.metric));
+        const float seed = ok ? (float)s64 : 0.0f;
+        const bool gfin = ft_reverse<float, BIG>(rc, d.nrev, seed, tape, X, lane, (float *)nullptr);
+        const bool use = ok && gfin;
+        if (polish) {
+            if (live && !ok) bad = 1;
+            if (ok && !gfin) ++skip;
+            if (ok) lsum += wb * rc_row_rho(r64, d.metric);
+        } else {
+            if (live && !use) bad = 1;
+            if (use) lsum += wb * rc_row_rho(r64, d.metric);
+        }
+        if (__any(ok && !gfin)) {
+            if (!use)
+                for (int j = 0; j < k; ++j) X[j * XS + lane] = 0.0f;
+        }
+        const double rs = s64 * r64;
+        const float rhi = (float)rs;
+        const float rlo = (float)(rs - (double)rhi);
+        X[k * XS + lane] = use ? rhi : 0.0f;
+        X[(k + 1) * XS + lane] = use ? rlo : 0.0f;
+        __syncthreads();
+        const int col = lane & 15;
+        const int sub = lane >> 4;
+#pragma unroll 4
+        for (int q = 0; q < 16; ++q) {
+            const float a0 = X[col * XS + 4 * q + sub];
+            acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, a0, acc00, 0, 0, 0);
+            if (KC == 32) {
+                const float a1 = X[(16 + col) * XS + 4 * q + sub];
+                acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, a0, acc10, 0, 0, 0);
+                acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, a1, acc11, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+        if ((step & 3) == 3) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                g00[r] += (double)acc00[r];
+                acc00[r] = 0.0f;
+                if (KC == 32) {
+                    g10[r] += (double)acc10[r];
+                    g11[r] += (double)acc11[r];
+                    acc10[r] = 0.0f;
+                    acc11[r] = 0.0f;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        g00[r] += (double)acc00[r];
+        if (KC == 32) {
+            g10[r] += (double)acc10[r];
+            g11[r] += (double)acc11[r];
+        }
+    }
+    /* Lane l, register r holds D[4*(l/16) + r][l % 16] of each 16x16 block. */
+    const int K = k + 2;
+    double *pg = pgram + (size_t)item * (size_t)tri_stride;
+    const int j = lane & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = 4 * (lane >> 4) + r;
+        if (i >= j && i < K) pg[i * (i + 1) / 2 + j] = g00[r];
+        if (KC == 32) {
+            const int i1 = 16 + i;
+            if (i1 < K) pg[i1 * (i1 + 1) / 2 + j] = g10[r];
+            if (i >= j && i1 < K) pg[i1 * (i1 + 1) / 2 + 16 + j] = g11[r];
+        }
+    }
+    const double ls = rc_wave_sum_f64(lsum);
+    int sk = skip;
+    for (int off = 32; off > 0; off >>= 1) sk += __shfl_xor(sk, off, 64);
+    const int anybad = __any(bad) ? 1 : 0;
+    if (lane == 0) {
+        ploss[item] = ls;
+        pbad[item] = anybad;
+        pskip[item] = sk;
+    }
+}
+
+/* ---- trial / exact loss: forward only ------------------------------------ */
+
+template <typename T, bool BIG>
+__global__ __launch_bounds__(64) void ft_trial_kernel(
+    const FtItem *__restrict__ items, const int *__restrict__ list, int nlist,
+    const FtCand *__restrict__ cands, const FtRows *__restrict__ rowsd,
+    const uint32_t *__restrict__ code, const int *__restrict__ run, const T *__restrict__ cp,
+    double *__restrict__ ploss)
+{
+    const int li = blockIdx.x;
+    if (li >= nlist) return;
+    const int item = list[li];
+    const FtItem it = items[item];
+    const int cand = it.cand;
+    if (!run[cand]) return;
+    const FtCand d = cands[cand];
+    const FtRows R = rowsd[d.rowset];
+    const int lane = threadIdx.x;
+    const T *cst = cp + (size_t)cand * FT_KMAX;
+    const uint32_t *fc = code + d.fwd_off;
+    const T *cols;
+    if (sizeof(T) == 4) cols = (const T *)(const void *)R.cols;
+    else cols = (const T *)(const void *)R.cols64;
+    double lsum = 0.0;
+    const int row_end = it.row0 + it.rows;
+    for (int base = it.row0; base < row_end; base += 64) {
+        const int row = base + lane;
+        bool live = row < row_end;
+        const int rr = live ? row : it.row0;
+        const double wb = R.w64 ? R.w64[rr] : 1.0;
+        live = live && (wb > 0.0);
+        typename XVec<T>::type xv;
+        load_x(cols, R.n, R.v, rr, xv);
+        Tape2<T, BIG> tape;
+        const T pred = ft_forward<T, BIG, false>(fc, d.m, cst, xv, cols, R.n, rr, tape);
+        const double r = (double)pred - R.y64[rr];
+        if (!live) continue;
+        bool hard = !isfinite(pred) || !isfinite(r);
+        if (sizeof(T) == 4) hard = hard || fabs((double)pred) > 1e30;
+        lsum += hard ? 1e30 : wb * rc_row_rho(r, d.metric);
+    }
+    const double ls = rc_wave_sum_f64(lsum);
+    if (lane == 0) ploss[item] = ls;
+}
+
+/* ---- per-row probe (test hook): pred, constant and input gradients -------- */
+
+__global__ __launch_bounds__(64) void ft_probe_kernel(const FtCand *__restrict__ cands,
+                                                      const FtRows *__restrict__ rowsd,
+                                                      const uint32_t *__restrict__ code,
+                                                      const float *__restrict__ cp32, int cand,
+                                                      int rows, float *__restrict__ out_pred,
+                                                      float *__restrict__ out_jac, int kmax,
+                                                      float *__restrict__ out_gx)
+{
+    __shared__ float X[32 * XS];
+    const int lane = threadIdx.x;
+    const FtCand d = cands[cand];
+    const FtRows R = rowsd[d.rowset];
+    const int row = blockIdx.x * 64 + lane;
+    const bool live = row < rows && row < R.n;
+    const int rr = live ? row : 0;
+    for (int i = lane; i < 32 * XS; i += 64) X[i] = 0.0f;
+    __syncthreads();
+    fv8 xv;
+    load_x(R.cols, R.n, R.v, rr, xv);
+    Tape2<float, true> tape;
+    const float pred = ft_forward<float, true, true>(code + d.fwd_off, d.m, cp32 + (size_t)cand * FT_KMAX,
+                                                     xv, R.cols, R.n, rr, tape);
+    float gx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    ft_reverse<float, true>(code + d.rev_off, d.nrev, 1.0f, tape, X, lane, gx);
+    __syncthreads();
+    if (!live) return;
+    out_pred[row] = pred;
+    for (int j = 0; j < d.k; ++j) out_jac[(size_t)row * kmax + j] = X[j * XS + lane];
+    for (int v = 0; v < R.v && v < 8; ++v) out_gx[(size_t)row * R.v + v] = gx[v];
+}
+
+/* ---- control: one wave per candidate --------------------------------------- */
+
+__device__ __forceinline__ double clamp_step(double t) { return fmin(1e6, fmax(-1e6, t)); }
+
+__global__ __launch_bounds__(64) void ft_ctrl_kernel(
+    int op, int ncand, FtState *__restrict__ S, const FtCand *__restrict__ cands,
+    const double *__restrict__ c0, int kmax, int *__restrict__ run_asm, int *__restrict__ run_trial,
+    float *__restrict__ cp32, double *__restrict__ cp64, const double *__restrict__ pgram,
+    int tri_stride, const double *__restrict__ ploss, const int *__restrict__ pbad,
+    const int *__restrict__ pskip, double *__restrict__ outc, double *__restrict__ outsse,
+    int *__restrict__ outstatus)
+{
+    const int cand = blockIdx.x;
+    if (cand >= ncand) return;
+    const int lane = threadIdx.x;
+    FtState *s = S + cand;
+    const FtCand d = cands[cand];
+    __shared__ double G[FT_TRI_MAX];
+    __shared__ double L[FT_KMAX * FT_KMAX];
+    __shared__ double sh_loss;
+    __shared__ int sh_bad, sh_skip;
+
+    const int ran_asm = run_asm[cand];
+    const int ran_trial = run_trial[cand];
+    const int k = d.k;
+    const int K = k + 2;
+    const int tri = K * (K + 1) / 2;
+
+    /* Fixed-order reductions over this candidate's items. */
+    const bool want_gram = (op == CT_AFTER_ASM || op == CT_AFTER_ASM_P) && ran_asm &&
+                           (s->stage == ST_NEED_ASM || s->stage == ST_P_NEED_ASM);
+    const bool want_loss = ((op == CT_AFTER_TRIAL || op == CT_AFTER_TRIAL_P || op == CT_REBASE) &&
+                            ran_trial) || want_gram;
+    if (want_gram) {
+        for (int q = lane; q < tri; q += 64) {
+            double acc = 0.0;
+            for (int t = 0; t < d.nitems; ++t)
+                acc += pgram[(size_t)(d.item0 + t) * (size_t)tri_stride + q];
+            G[q] = acc;
+        }
+    }
+    if (lane == 0 && want_gram) s->n_asm += 1;
+    if (lane == 0 && want_loss && !want_gram) {
+        if (op == CT_AFTER_TRIAL) s->n_trial += 1;
+        else s->n_exact += 1;
+    }
+    if (want_loss && lane == 0) {
+        double acc = 0.0;
+        int b = 0, sk = 0;
+        for (int t = 0; t < d.nitems; ++t) {
+            acc += ploss[d.item0 + t];
+            if (want_gram) {
+                b |= pbad[d.item0 + t];
+                sk += pskip[d.item0 + t];
+            }
+        }
+        sh_loss = acc;
+        sh_bad = b;
+        sh_skip = sk;
+    }
+    __syncthreads();
+    if (lane != 0) return;
+
+    double *cc = s->c;
+    auto set_pass32 = [&](const double *v) {
+        for (int j = 0; j < FT_KMAX; ++j) cp32[(size_t)cand * FT_KMAX + j] = j < k ? (float)v[j] : 0.0f;
+    };
+    auto set_pass64 = [&](const double *v) {
+        for (int j = 0; j < FT_KMAX; ++j) cp64[(size_t)cand * FT_KMAX + j] = j < k ? v[j] : 0.0;
+    };
+    /* Solve at the cached normal equations and stage the trial. */
+    auto solve_to_trial = [&](double lambda) -> int {
+        double delta[FT_KMAX];
+        const int rc = ft_solve(s->jtj, s->jtr, lambda, k, delta, L);
+        if (rc == 1) return 1;
+        if (rc == 4) s->srank = 1;
+        for (int j = 0; j < k; ++j) s->trial[j] = clamp_step(cc[j] + delta[j]);
+        return 0;
+    };
+    auto take_gram = [&]() {
+        for (int i = 0; i < k; ++i) {
+            for (int j = 0; j <= i; ++j) s->jtj[i * (i + 1) / 2 + j] = G[i * (i + 1) / 2 + j];
+            s->jtr[i] = G[k * (k + 1) / 2 + i] + G[(k + 1) * (k + 2) / 2 + i];
+        }
+    };
+    auto lm_next = [&]() {
+        /* After an iteration: start the next one or end the loop. */
+        if (s->stage == ST_LM_DONE) return;
+        if (s->assembled) {
+            if (solve_to_trial(s->lambda) != 0) {
+                s->status = 1;
+                s->stage = ST_LM_DONE;
+                return;
+            }
+            set_pass32(s->trial);
+            s->halvings = s->irls ? 2 : 0;
+            s->stage = ST_NEED_TRIAL;
+        } else {
+            set_pass32(cc);
+            s->stage = ST_NEED_ASM;
+        }
+    };
+    auto end_inner = [&]() {
+        /* The inner loop of this round ended (count or break). */
+        s->round += 1;
+        s->it = 0;
+        if (s->round >= s->rounds || s->best == 0.0 || s->status != 0) {
+            s->stage = ST_LM_DONE;
+        } else {
+            s->assembled = 0; /* new IRLS weights, new Gram */
+        }
+    };
+
+    switch (op) {
+        case CT_INIT: {
+            s->k = k;
+            s->metric = d.metric;
+            s->irls = (d.metric == RC_LOSS_ABSOLUTE);
+            const int it = s->iters;
+            s->rounds = s->irls ? (it < 1 ? 1 : (it < 4 ? it : 4)) : 1;
+            s->inner = s->irls ? (it / s->rounds < 1 ? 1 : it / s->rounds) : it;
+            for (int j = 0; j < FT_KMAX; ++j) cc[j] = j < k ? c0[(size_t)cand * kmax + j] : 0.0;
+            s->lambda = 1e-3;
+            s->best = 0.0;
+            s->status = 0;
+            s->sstart = 0;
+            s->srank = 0;
+            s->improved = 0;
+            s->skipped = 0;
+            s->it = 0;
+            s->round = 0;
+            s->assembled = 0;
+            s->first = 1;
+            s->halvings = 0;
+            s->polish_it = 0;
+            s->n_asm = 0;
+            s->n_trial = 0;
+            s->n_exact = 0;
+            if (s->pre_status >= 0) {
+                s->stage = ST_DONE;
+            } else if (s->iters > 0) {
+                set_pass32(cc);
+                s->stage = ST_NEED_ASM;
+            } else {
+                s->stage = ST_LM_DONE;
+            }
+            break;
+        }
+        case CT_AFTER_ASM: {
+            if (!want_gram) break;
+            const double loss = sh_loss;
+            if (s->first) {
+                s->first = 0;
+                s->best = loss;
+                s->sstart = sh_bad ? 0 : 1;
+                if (sh_bad) { s->status = 2; s->stage = ST_LM_DONE; break; }
+                if (s->best == 0.0) { s->stage = ST_LM_DONE; break; }
+            } else if (sh_bad) {
+                s->status = 2;
+                s->stage = ST_LM_DONE;
+                break;
+            }
+            take_gram();
+            s->assembled = 1;
+            lm_next();
+            break;
+        }
+        case CT_AFTER_TRIAL: {
+            if (!ran_trial || s->stage != ST_NEED_TRIAL) break;
+            const double tsse = sh_loss;
+            if (!(tsse < s->best) && s->halvings > 0) {
+                /* True-metric Armijo halving for the L1 arm (eval.cu:2496-2509). */
+                s->halvings -= 1;
+                for (int j = 0; j < k; ++j) s->trial[j] = cc[j] + 0.5 * (s->trial[j] - cc[j]);
+                set_pass32(s->trial);
+                break;
+            }
+            bool brk = false;
+            if (tsse < s->best) {
+                for (int j = 0; j < k; ++j) cc[j] = s->trial[j];
+                s->best = tsse;
+                s->lambda = fmax(s->lambda / 8.0, 1e-12);
+                s->improved = 1;
+                s->assembled = 0;
+                if (s->best == 0.0) brk = true;
+            } else {
+                s->lambda *= 6.0;
+                if (s->lambda > 1e9) brk = true;
+            }
+            s->it += 1;
+            if (brk || s->it >= s->inner) end_inner();
+            lm_next();
+            break;
+        }
+        case CT_PRE_REBASE: {
+            if (s->stage != ST_DONE) s->stage = ST_LM_DONE;
+            if (s->stage == ST_LM_DONE && s->status == 0) set_pass64(cc);
+            break;
+        }
+        case CT_REBASE: {
+            if (s->stage != ST_LM_DONE) break;
+            if (ran_trial && s->status == 0 && isfinite(sh_loss)) s->best = sh_loss;
+            if (s->iters > 0 && s->status == 0 && s->best != 0.0) {
+                s->polish_it = 0;
+                set_pass32(cc);
+                set_pass64(cc);
+                s->stage = ST_P_NEED_ASM;
+            } else {
+                s->stage = ST_DONE;
+            }
+            break;
+        }
+        case CT_AFTER_ASM_P: {
+            if (!want_gram) break;
+            if (sh_bad) { s->status = 2; s->stage = ST_DONE; break; }
+            s->skipped += sh_skip;
+            take_gram();
+            if (solve_to_trial(0.0) != 0) { s->stage = ST_DONE; break; }
+            set_pass64(s->trial);
+            s->halvings = s->irls ? 1 : 0;
+            s->stage = ST_P_NEED_TRIAL;
+            break;
+        }
+        case CT_AFTER_TRIAL_P: {
+            if (!ran_trial || s->stage != ST_P_NEED_TRIAL) break;
+            const double tsse = sh_loss;
+            if (!(tsse < s->best) && s->halvings > 0) {
+                s->halvings -= 1;
+                for (int j = 0; j < k; ++j) s->trial[j] = cc[j] + 0.5 * (s->trial[j] - cc[j]);
+                set_pass64(s->trial);
+                break;
+            }
+            if (tsse < s->best) {
+                for (int j = 0; j < k; ++j) cc[j] = s->trial[j];
+                s->best = tsse;
+                s->improved = 1;
+                s->polish_it += 1;
+                if (s->polish_it < 4 && s->best != 0.0) {
+                    set_pass32(cc);
+                    set_pass64(cc);
+                    s->stage = ST_P_NEED_ASM;
+                } else {
+                    s->stage = ST_DONE;
+                }
+            } else {
+                s->stage = ST_DONE;
+            }
+            break;
+        }
+        case CT_FINAL: {
+            int st = s->status;
+            if (s->pre_status >= 0) {
+                st = s->pre_status;
+            } else if (st == 1 || st == 2) {
+                st = !s->sstart ? 7 : (s->improved ? 5 : 6);
+            } else if (st == 0) {
+                if (s->skipped > 0) st = 8;
+                else if (s->srank) st = 4;
+                else if (!s->improved && s->iters > 0) st = 6;
+            }
+            for (int j = 0; j < kmax; ++j)
+                outc[(size_t)cand * kmax + j] = j < k ? cc[j] : c0[(size_t)cand * kmax + j];
+            outsse[cand] = s->pre_status >= 0 ? (s->pre_status == 7 ? 0.0 : __builtin_nan(""))
+                                              : s->best;
+            outstatus[cand] = st;
+            break;
+        }
+        default: break;
+    }
+    /* Flags for the next pass. */
+    const int stg = s->stage;
+    switch (op) {
+        case CT_INIT:
+        case CT_AFTER_ASM:
+        case CT_AFTER_TRIAL:
+            run_asm[cand] = (stg == ST_NEED_ASM);
+            run_trial[cand] = (stg == ST_NEED_TRIAL);
+            break;
+        case CT_PRE_REBASE:
+            run_asm[cand] = 0;
+            run_trial[cand] = (stg == ST_LM_DONE && s->status == 0);
+            break;
+        case CT_REBASE:
+        case CT_AFTER_ASM_P:
+        case CT_AFTER_TRIAL_P:
+            run_asm[cand] = (stg == ST_P_NEED_ASM);
+            run_trial[cand] = (stg == ST_P_NEED_TRIAL);
+            break;
+        default:
+            run_asm[cand] = 0;
+            run_trial[cand] = 0;
+            break;
+    }
+}
+
+__global__ void ft_state_init(FtState *S, int ncand, const int *pre_status, int iters)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncand) return;
+    S[c].pre_status = pre_status[c];
+    S[c].iters = iters;
+}
+
+} /* namespace */
+
+/* ======================================================================== */
+/* Host side.                                                                */
+
+/* Operators with an analytic value/adjoint pair (fit.rs:890-920). */
+static bool fit_op_ok(uint32_t op)
+{
+    switch (op) {
+        case OP_CONST: case OP_VAR: case OP_ADD: case OP_SUB: case OP_MUL: case OP_DIV:
+        case OP_SIN: case OP_COS: case OP_TAN: case OP_EXP: case OP_LOG: case OP_SQRT:
+        case OP_ABS: case OP_TANH: case OP_SIG: case OP_SQUARE: case OP_CUBE: case OP_NEG:
+        case OP_GAUSS: case OP_POW: case OP_SINH: case OP_COSH: case OP_ERF: case OP_ERFC:
+            return true;
+        default: return false;
+    }
+}
+
+static bool is_bin(uint32_t op)
+{
+    return op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV || op == OP_POW;
+}
+
+/* Per-candidate translation result. */
+struct FtProg {
+    std::vector<uint32_t> fwd, rev;
+    int pre_status = -1; /* >= 0: decided on the host (no device work) */
+    int max_stack = 0, max_adj = 0, tape = 0, tape_l = 0, tape_r = 0;
+};
+
+/* Translate one candidate's records.  want_const / want_var select which
+   leaves the reverse sweep must reach. */
+static FtProg translate_fit(const uint32_t *w, int words, int p, int nvars, bool want_const,
+                            bool want_var)
+{
+    FtProg out;
+    const int m = words / 4;
+    if (words % 4 != 0 || m < 1 || m > FT_MAX_NODES || p < 0 || p > FT_KMAX) {
+        out.pre_status = 1;
+        return out;
+    }
+    std::vector<int> st;
+    std::vector<char> onpath(m, 0);
+    bool unsupported = false;
+    for (int i = 0; i < m; ++i) {
+        const uint32_t op = w[4 * i], k0 = w[4 * i + 1], k1 = w[4 * i + 2], slot = w[4 * i + 3];
+        if (!fit_op_ok(op)) unsupported = true;
+        if (op == OP_CONST) {
+            if ((int)slot >= p) { out.pre_status = 1; return out; }
+            onpath[i] = want_const;
+            st.push_back(i);
+        } else if (op == OP_VAR) {
+            if ((int)slot >= nvars || slot > 0xffffu) { out.pre_status = 1; return out; }
+            onpath[i] = want_var;
+            st.push_back(i);
+        } else if (is_bin(op)) {
+            if (st.size() < 2 || st.back() != (int)k1 || st[st.size() - 2] != (int)k0) {
+                out.pre_status = 1; /* not the canonical post-order tree */
+                return out;
+            }
+            st.pop_back();
+            st.pop_back();
+            onpath[i] = onpath[k0] || onpath[k1];
+            st.push_back(i);
+        } else {
+            if (st.empty() || st.back() != (int)k0) { out.pre_status = 1; return out; }
+            st.pop_back();
+            onpath[i] = onpath[k0];
+            st.push_back(i);
+        }
+    }
+    if (st.size() != 1 || st[0] != m - 1) { out.pre_status = 1; return out; }
+    if (unsupported) {
+        /* The reference device forward yields NaN for every row: a non-finite
+           start, status 7, constants unchanged. */
+        out.pre_status = 7;
+        return out;
+    }
+    /* Forward stream, stack depth and tape size. */
+    int depth = 0, tape = 0;
+    for (int i = 0; i < m; ++i) {
+        const uint32_t op = w[4 * i], k0 = w[4 * i + 1], k1 = w[4 * i + 2], slot = w[4 * i + 3];
+        uint32_t fl = 0, arg = 0;
+        if (op == OP_CONST || op == OP_VAR) {
+            arg = slot;
+            ++depth;
+        } else if (is_bin(op)) {
+            if (onpath[k0]) fl |= FF_D0;
+            if (onpath[k1]) fl |= FF_D1;
+            /* ADD/SUB partials are the constants +-1: no tape. */
+            if (op == OP_ADD || op == OP_SUB) fl = 0;
+            --depth;
+        } else {
+            if (onpath[k0] && op != OP_NEG) fl |= FF_D0;
+        }
+        tape += ((fl & FF_D0) ? 1 : 0) + ((fl & FF_D1) ? 1 : 0);
+        out.tape_l += (fl & FF_D0) ? 1 : 0;
+        out.tape_r += (fl & FF_D1) ? 1 : 0;
+        out.max_stack = std::max(out.max_stack, depth);
+        out.fwd.push_back(op | (fl << 8) | (arg << 16));
+    }
+    out.tape = tape;
+    /* Reverse stream over on-path nodes in reverse post-order. */
+    int ad = 1, adpeak = 1;
+    for (int i = m - 1; i >= 0; --i) {
+        if (!onpath[i]) continue;
+        const uint32_t op = w[4 * i], k0 = w[4 * i + 1], k1 = w[4 * i + 2], slot = w[4 * i + 3];
+        uint32_t kind, fl = 0, arg = 0;
+        --ad;
+        if (op == OP_CONST) { kind = RK_CONST; arg = slot; }
+        else if (op == OP_VAR) { kind = RK_VAR; arg = slot; }
+        else if (is_bin(op)) {
+            kind = (op == OP_ADD) ? RK_ADD : (op == OP_SUB) ? RK_SUB : RK_BINARY;
+            if (onpath[k0]) { fl |= RF_LEFT; ++ad; }
+            if (onpath[k1]) { fl |= RF_RIGHT; ++ad; }
+        } else {
+            kind = (op == OP_NEG) ? RK_NEG : RK_UNARY;
+            ++ad;
+        }
+        adpeak = std::max(adpeak, ad);
+        out.rev.push_back(kind | (fl << 8) | (arg << 16));
+    }
+    out.max_adj = adpeak;
+    return out;
+}
+
+/* Device-resident state of the fit arm (grown at the high-water mark). */
+struct RcFitState {
+    unsigned char *dev = nullptr;
+    size_t dev_bytes = 0;
+    unsigned char *host = nullptr;
+    size_t host_bytes = 0;
+    int mode = 0; /* 0 mixed, 1 plain f64 */
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> graph_key;
+    /* Profiling (rc_fit_profile): per-pass event timing, launches eagerly. */
+    int profile = 0;
+    double t_pass[4] = {0, 0, 0, 0};  /* asm, trial32, exact64, ctrl */
+    int n_pass[4] = {0, 0, 0, 0};
+    FtState *last_state = nullptr;
+    int last_nprog = 0;
+};
+
+void rc_fit_state_free(RcCtx *c)
+{
+    if (!c->fit) return;
+    if (c->fit->exec) (void)hipGraphExecDestroy(c->fit->exec);
+    if (c->fit->graph) (void)hipGraphDestroy(c->fit->graph);
+    if (c->fit->dev) (void)hipFree(c->fit->dev);
+    if (c->fit->host) (void)hipHostFree(c->fit->host);
+    delete c->fit;
+    c->fit = nullptr;
+}
+
+int rc_rows_ensure_f64(RcCtx *c, RcRows *r);
+int rc_fit64_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const int *pcount, int nprog,
+                 int kmax, const double *c0, const std::vector<int> &rowset,
+                 const std::vector<int> &metric, const std::vector<int> &pre, int iters,
+                 double *outc, double *outsse, int *outstatus);
+
+static size_t take(size_t &cur, size_t bytes)
+{
+    size_t o = (cur + 255) & ~(size_t)255;
+    cur = o + bytes;
+    return o;
+}
+
+static int fit_reserve(RcCtx *c, size_t dev_bytes, size_t host_bytes)
+{
+    RcFitState *f = c->fit;
+    if (dev_bytes > f->dev_bytes) {
+        if (f->dev) { (void)hipFree(f->dev); c->fs_alloc_calls++; if (c->fs_batches) c->fs_steady_alloc_calls++; }
+        f->dev = nullptr;
+        size_t want = dev_bytes + dev_bytes / 4;
+        if (hipMalloc((void **)&f->dev, want) != hipSuccess) {
+            (void)hipGetLastError();
+            f->dev = nullptr;
+            f->dev_bytes = 0;
+            return -1;
+        }
+        f->dev_bytes = want;
+        c->fs_alloc_calls++;
+        c->fs_growths++;
+        if (c->fs_batches) c->fs_steady_alloc_calls++;
+        /* A new region invalidates the recorded launch sequence. */
+        if (f->exec) { (void)hipGraphExecDestroy(f->exec); f->exec = nullptr; }
+        if (f->graph) { (void)hipGraphDestroy(f->graph); f->graph = nullptr; }
+        f->graph_key.clear();
+    }
+    if (host_bytes > f->host_bytes) {
+        if (f->host) { (void)hipHostFree(f->host); c->fs_alloc_calls++; }
+        f->host = nullptr;
+        size_t want = host_bytes + host_bytes / 4;
+        if (hipHostMalloc((void **)&f->host, want, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            f->host = nullptr;
+            f->host_bytes = 0;
+            return -1;
+        }
+        f->host_bytes = want;
+        c->fs_alloc_calls++;
+        c->fs_growths++;
+        if (c->fs_batches) c->fs_steady_alloc_calls++;
+    }
+    return 0;
+}
+
+/* Class of a candidate's interpreter storage. */
+static bool prog_is_big(const FtProg &p)
+{
+    return p.max_stack + 1 > SMALL_S || p.max_adj + 2 > SMALL_AS || p.tape_l + 1 > SMALL_TAPE ||
+           p.tape_r + 1 > SMALL_TAPE;
+}
+
+/* The shared driver of rc_fit_batch / rc_fit_batch_hetero. */
+static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const int *pcount,
+                     int nprog, int kmax, const double *c0, const int *rowset_in,
+                     const int *metric_in, int metric_all, int iters, double *outc,
+                     double *outsse, int *outstatus)
+{
+    if (!c->fit) c->fit = new RcFitState();
+    RcFitState *f = c->fit;
+    const double t_begin = rc_now_ms();
+    /* Host translation. */
+    std::vector<FtProg> progs((size_t)nprog);
+    std::vector<int> rowset((size_t)nprog), metric((size_t)nprog);
+    bool any_irls = false;
+    for (int i = 0; i < nprog; ++i) {
+        const int rs = rowset_in ? rowset_in[i] : 0;
+        const int mt = metric_in ? metric_in[i] : metric_all;
+        if (rs < 0 || rs >= c->nsets) return -1.0f;
+        if (mt != RC_LOSS_SQUARED && mt != RC_LOSS_LOG_SQUASHED && mt != RC_LOSS_ABSOLUTE) return -1.0f;
+        rowset[i] = rs;
+        metric[i] = mt;
+        if (mt == RC_LOSS_ABSOLUTE) any_irls = true;
+        const int words = (int)(off[i + 1] - off[i]);
+        if (pcount[i] < 1 || pcount[i] > kmax) {
+            progs[i].pre_status = 1;
+            continue;
+        }
+        progs[i] = translate_fit(prog + off[i], words, pcount[i], c->sets[rs].v, true, false);
+    }
+    for (int s = 0; s < c->nsets; ++s)
+        if (rc_rows_ensure_f64(c, &c->sets[s]) != 0) return -1.0f;
+
+    if (f->mode == 1) {
+        std::vector<int> pre64((size_t)nprog);
+        for (int i = 0; i < nprog; ++i) pre64[i] = progs[i].pre_status;
+        if (rc_fit64_run(c, prog, off, pcount, nprog, kmax, c0, rowset, metric, pre64, iters, outc,
+                         outsse, outstatus) != 0)
+            return -1.0f;
+        c->fs_batches++;
+        return (float)c->stats.kernel_ms;
+    }
+
+    /* Candidate descriptors, streams and work items. */
+    std::vector<FtCand> cands((size_t)nprog);
+    std::vector<uint32_t> code;
+    std::vector<FtItem> items;
+    std::vector<int> pre((size_t)nprog, -1);
+    std::vector<int> list[4]; /* (KC16,small) (KC32,small) (KC16,big) (KC32,big) */
+    std::vector<int> tlist[2];
+    int maxK = 2;
+    for (int i = 0; i < nprog; ++i) {
+        FtCand &d = cands[i];
+        memset(&d, 0, sizeof(d));
+        d.rowset = rowset[i];
+        d.metric = metric[i];
+        pre[i] = progs[i].pre_status;
+        /* A host-decided candidate keeps k = 0: its outputs are c0 verbatim. */
+        d.k = (pre[i] >= 0) ? 0 : pcount[i];
+        if (pre[i] >= 0) continue;
+        d.fwd_off = (uint32_t)code.size();
+        code.insert(code.end(), progs[i].fwd.begin(), progs[i].fwd.end());
+        d.rev_off = (uint32_t)code.size();
+        code.insert(code.end(), progs[i].rev.begin(), progs[i].rev.end());
+        d.m = (int)progs[i].fwd.size();
+        d.nrev = (int)progs[i].rev.size();
+        const int n = c->sets[d.rowset].n;
+        d.item0 = (int)items.size();
+        const bool big = prog_is_big(progs[i]);
+        const int cls = (d.k + 2 <= 16 ? 0 : 1) + (big ? 2 : 0);
+        maxK = std::max(maxK, d.k + 2);
+        for (int r0 = 0; r0 < n; r0 += TILE_ROWS) {
+            FtItem it;
+            it.cand = i;
+            it.row0 = r0;
+            it.rows = std::min(TILE_ROWS, n - r0);
+            it.pad = 0;
+            list[cls].push_back((int)items.size());
+            tlist[big ? 1 : 0].push_back((int)items.size());
+            items.push_back(it);
+        }
+        d.nitems = (int)items.size() - d.item0;
+    }
+    if (code.empty()) code.push_back(0);
+    const int nitems = (int)items.size();
+    const int tri_stride = maxK * (maxK + 1) / 2;
+    /* Longest items first within each class list: fewer stragglers. */
+    for (int q = 0; q < 4; ++q)
+        std::stable_sort(list[q].begin(), list[q].end(), [&](int a, int b) {
+            return (long long)cands[items[a].cand].m * items[a].rows >
+                   (long long)cands[items[b].cand].m * items[b].rows;
+        });
+    for (int q = 0; q < 2; ++q)
+        std::stable_sort(tlist[q].begin(), tlist[q].end(), [&](int a, int b) {
+            return (long long)cands[items[a].cand].m * items[a].rows >
+                   (long long)cands[items[b].cand].m * items[b].rows;
+        });
+    std::vector<FtRows> rowsd((size_t)c->nsets);
+    for (int s = 0; s < c->nsets; ++s) {
+        rowsd[s].cols = c->sets[s].cols;
+        rowsd[s].cols64 = c->sets[s].cols64;
+        rowsd[s].y64 = c->sets[s].y64;
+        rowsd[s].w64 = c->sets[s].w64;
+        rowsd[s].n = c->sets[s].n;
+        rowsd[s].v = c->sets[s].v;
+    }
+    std::vector<int> flat_lists;
+    int lofs[6];
+    for (int q = 0; q < 4; ++q) {
+        lofs[q] = (int)flat_lists.size();
+        flat_lists.insert(flat_lists.end(), list[q].begin(), list[q].end());
+    }
+    lofs[4] = (int)flat_lists.size();
+    for (int q = 0; q < 2; ++q) flat_lists.insert(flat_lists.end(), tlist[q].begin(), tlist[q].end());
+    lofs[5] = (int)flat_lists.size();
+    if (flat_lists.empty()) flat_lists.push_back(0);
+
+    /* Region layout: inputs [code | cands | rows | items | lists | pre | c0]
+       then outputs [c | sse | status] then work. */
+    size_t cur = 0;
+    const size_t o_code = take(cur, code.size() * 4);
+    const size_t o_cand = take(cur, cands.size() * sizeof(FtCand));
+    const size_t o_rows = take(cur, rowsd.size() * sizeof(FtRows));
+    const size_t o_items = take(cur, std::max<size_t>(1, items.size()) * sizeof(FtItem));
+    const size_t o_lists = take(cur, flat_lists.size() * 4);
+    const size_t o_pre = take(cur, (size_t)nprog * 4);
+    const size_t o_c0 = take(cur, (size_t)nprog * kmax * 8);
+    const size_t in_end = take(cur, 0);
+    const size_t o_outc = take(cur, (size_t)nprog * kmax * 8);
+    const size_t o_sse = take(cur, (size_t)nprog * 8);
+    const size_t o_st = take(cur, (size_t)nprog * 4);
+    const size_t io_end = take(cur, 0);
+    const size_t o_state = take(cur, (size_t)nprog * sizeof(FtState));
+    const size_t o_run_a = take(cur, (size_t)nprog * 4);
+    const size_t o_run_t = take(cur, (size_t)nprog * 4);
+    const size_t o_cp32 = take(cur, (size_t)nprog * FT_KMAX * 4);
+    const size_t o_cp64 = take(cur, (size_t)nprog * FT_KMAX * 8);
+    const size_t o_pgram = take(cur, (size_t)std::max(1, nitems) * tri_stride * 8);
+    const size_t o_ploss = take(cur, (size_t)std::max(1, nitems) * 8);
+    const size_t o_pbad = take(cur, (size_t)std::max(1, nitems) * 4);
+    const size_t o_pskip = take(cur, (size_t)std::max(1, nitems) * 4);
+    const size_t dev_end = take(cur, 0);
+    double t = rc_now_ms();
+    if (fit_reserve(c, dev_end, io_end) != 0) return -1.0f;
+    c->stats.alloc_ms += rc_now_ms() - t;
+
+    unsigned char *H = f->host;
+    unsigned char *D = f->dev;
+    memcpy(H + o_code, code.data(), code.size() * 4);
+    memcpy(H + o_cand, cands.data(), cands.size() * sizeof(FtCand));
+    memcpy(H + o_rows, rowsd.data(), rowsd.size() * sizeof(FtRows));
+    if (!items.empty()) memcpy(H + o_items, items.data(), items.size() * sizeof(FtItem));
+    memcpy(H + o_lists, flat_lists.data(), flat_lists.size() * 4);
+    memcpy(H + o_pre, pre.data(), (size_t)nprog * 4);
+    memcpy(H + o_c0, c0, (size_t)nprog * kmax * 8);
+    t = rc_now_ms();
+    hipStream_t s = c->s_fit;
+    if (hipMemcpyAsync(D, H, in_end, hipMemcpyHostToDevice, s) != hipSuccess) return -1.0f;
+    c->stats.h2d_ms += rc_now_ms() - t;
+    c->stats.h2d_bytes += in_end;
+
+    const uint32_t *d_code = (const uint32_t *)(D + o_code);
+    const FtCand *d_cand = (const FtCand *)(D + o_cand);
+    const FtRows *d_rows = (const FtRows *)(D + o_rows);
+    const FtItem *d_items = (const FtItem *)(D + o_items);
+    const int *d_lists = (const int *)(D + o_lists);
+    const double *d_c0 = (const double *)(D + o_c0);
+    double *d_outc = (double *)(D + o_outc);
+    double *d_sse = (double *)(D + o_sse);
+    int *d_st = (int *)(D + o_st);
+    FtState *d_state = (FtState *)(D + o_state);
+    int *d_run_a = (int *)(D + o_run_a);
+    int *d_run_t = (int *)(D + o_run_t);
+    float *d_cp32 = (float *)(D + o_cp32);
+    double *d_cp64 = (double *)(D + o_cp64);
+    double *d_pgram = (double *)(D + o_pgram);
+    double *d_ploss = (double *)(D + o_ploss);
+    int *d_pbad = (int *)(D + o_pbad);
+    int *d_pskip = (int *)(D + o_pskip);
+
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+    auto mark = [&](int kind, auto &&fn) {
+        if (!f->profile) { fn(); return; }
+        hipEvent_t a, b;
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        (void)hipEventRecord(a, s);
+        fn();
+        (void)hipEventRecord(b, s);
+        marks.push_back({kind, {a, b}});
+    };
+    auto ctrl0 = [&](int op) {
+        ft_ctrl_kernel<<<nprog, 64, 0, s>>>(op, nprog, d_state, d_cand, d_c0, kmax, d_run_a, d_run_t,
+                                            d_cp32, d_cp64, d_pgram, tri_stride, d_ploss, d_pbad,
+                                            d_pskip, d_outc, d_sse, d_st);
+    };
+    auto ctrl = [&](int op) { mark(3, [&]() { ctrl0(op); }); };
+    auto asm_launch = [&](auto kern, int q) {
+        const int nl = lofs[q + 1] - lofs[q];
+        if (nl <= 0) return;
+        kern<<<nl, 64, 0, s>>>(d_items, d_lists + lofs[q], nl, d_cand, d_rows, d_code, d_run_a,
+                               d_cp32, d_cp64, d_pgram, tri_stride, d_ploss, d_pbad, d_pskip);
+    };
+    auto asm_pass0 = [&](int polish) {
+        if (polish) {
+            asm_launch(ft_asm_kernel<16, false, true>, 0);
+            asm_launch(ft_asm_kernel<32, false, true>, 1);
+            asm_launch(ft_asm_kernel<16, true, true>, 2);
+            asm_launch(ft_asm_kernel<32, true, true>, 3);
+        } else {
+            asm_launch(ft_asm_kernel<16, false, false>, 0);
+            asm_launch(ft_asm_kernel<32, false, false>, 1);
+            asm_launch(ft_asm_kernel<16, true, false>, 2);
+            asm_launch(ft_asm_kernel<32, true, false>, 3);
+        }
+    };
+    auto asm_pass = [&](int polish) { mark(0, [&]() { asm_pass0(polish); }); };
+    auto trial_pass0 = [&](bool exact) {
+        for (int q = 0; q < 2; ++q) {
+            const int a = (q == 0) ? lofs[4] : lofs[4] + (int)tlist[0].size();
+            const int nl = (q == 0) ? (int)tlist[0].size() : (int)tlist[1].size();
+            if (nl <= 0) continue;
+            const int *L = d_lists + a;
+            if (!exact && q == 0)
+                ft_trial_kernel<float, false><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
+            else if (!exact)
+                ft_trial_kernel<float, true><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
+            else if (q == 0)
+                ft_trial_kernel<double, false><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
+            else
+                ft_trial_kernel<double, true><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
+        }
+    };
+
+    auto trial_pass = [&](bool exact) { mark(exact ? 2 : 1, [&]() { trial_pass0(exact); }); };
+
+    /* The launch sequence: recorded as a hipGraph per batch shape. */
+    std::vector<uint64_t> key = {(uint64_t)nprog, (uint64_t)nitems, (uint64_t)iters,
+                                 (uint64_t)any_irls, (uint64_t)kmax, (uint64_t)tri_stride,
+                                 (uint64_t)dev_end, (uint64_t)(uintptr_t)D};
+    for (int q = 0; q < 6; ++q) key.push_back((uint64_t)lofs[q]);
+    const char *nograph = getenv("RCSR_FIT_NO_GRAPH");
+    const bool use_graph = !(nograph && nograph[0] == '1') && !f->profile;
+    auto record = [&]() {
+        ft_state_init<<<(nprog + 63) / 64, 64, 0, s>>>(d_state, nprog, (const int *)(D + o_pre), iters);
+        ctrl(CT_INIT);
+        for (int slot = 0; slot < iters; ++slot) {
+            asm_pass(0);
+            ctrl(CT_AFTER_ASM);
+            trial_pass(false);
+            ctrl(CT_AFTER_TRIAL);
+            if (any_irls) {
+                for (int h = 0; h < 2; ++h) {
+                    trial_pass(false);
+                    ctrl(CT_AFTER_TRIAL);
+                }
+            }
+        }
+        ctrl(CT_PRE_REBASE);
+        trial_pass(true);
+        ctrl(CT_REBASE);
+        for (int pit = 0; pit < 4; ++pit) {
+            asm_pass(1);
+            ctrl(CT_AFTER_ASM_P);
+            trial_pass(true);
+            ctrl(CT_AFTER_TRIAL_P);
+            if (any_irls) {
+                trial_pass(true);
+                ctrl(CT_AFTER_TRIAL_P);
+            }
+        }
+        ctrl(CT_FINAL);
+    };
+    t = rc_now_ms();
+    (void)hipEventRecord(c->t0, s);
+    if (use_graph) {
+        if (!f->exec || f->graph_key != key) {
+            if (f->exec) { (void)hipGraphExecDestroy(f->exec); f->exec = nullptr; }
+            if (f->graph) { (void)hipGraphDestroy(f->graph); f->graph = nullptr; }
+            hipStream_t cap;
+            if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) return -1.0f;
+            std::swap(s, cap);
+            bool okc = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (okc) {
+                record();
+                okc = hipStreamEndCapture(s, &f->graph) == hipSuccess;
+            }
+            std::swap(s, cap);
+            (void)hipStreamDestroy(cap);
+            if (!okc || hipGraphInstantiate(&f->exec, f->graph, nullptr, nullptr, 0) != hipSuccess) {
+                (void)hipGetLastError();
+                if (f->graph) { (void)hipGraphDestroy(f->graph); f->graph = nullptr; }
+                f->exec = nullptr;
+                return -1.0f;
+            }
+            f->graph_key = key;
+        }
+        if (hipGraphLaunch(f->exec, s) != hipSuccess) return -1.0f;
+    } else {
+        record();
+    }
+    (void)hipEventRecord(c->t1, s);
+    c->stats.launch_ms += rc_now_ms() - t;
+    if (hipGetLastError() != hipSuccess) return -1.0f;
+    t = rc_now_ms();
+    if (hipMemcpyAsync(H + in_end, D + in_end, io_end - in_end, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return -1.0f;
+    if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess) return -1.0f;
+    c->stats.sync_ms += rc_now_ms() - t;
+    for (auto &mk : marks) {
+        float e = 0.0f;
+        (void)hipEventElapsedTime(&e, mk.second.first, mk.second.second);
+        f->t_pass[mk.first] += e;
+        f->n_pass[mk.first] += 1;
+        (void)hipEventDestroy(mk.second.first);
+        (void)hipEventDestroy(mk.second.second);
+    }
+    f->last_state = d_state;
+    f->last_nprog = nprog;
+    memcpy(outc, H + o_outc, (size_t)nprog * kmax * 8);
+    memcpy(outsse, H + o_sse, (size_t)nprog * 8);
+    memcpy(outstatus, H + o_st, (size_t)nprog * 4);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, c->t0, c->t1);
+    c->stats.kernel_ms = ms;
+    c->stats.batch = nprog;
+    c->stats.wall_ms = rc_now_ms() - t_begin;
+    c->fs_batches++;
+    return ms;
+}
+
+extern "C" {
+
+float rc_fit_batch(void *ctx, const unsigned int *prog, const unsigned int *off, const int *pcount,
+                   int nprog, int kmax, const double *c0, const int *gauge, int iters, int metric,
+                   double *outc, double *outsse, int *outstatus)
+{
+    (void)gauge;
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || !prog || !off || !pcount || !c0 || nprog <= 0 || kmax <= 0 || iters < 0) return -1.0f;
+    if (!outc || !outsse || !outstatus) return -1.0f;
+    if (kmax > FT_KMAX) return -1.0f;
+    return fit_run(c, prog, off, pcount, nprog, kmax, c0, nullptr, nullptr, metric, iters, outc,
+                   outsse, outstatus);
+}
+
+float rc_fit_batch_hetero(void *ctx, const unsigned int *prog, const unsigned int *off,
+                          const int *pcount, int nprog, int kmax, const double *c0,
+                          const int *rowset, const int *metric, int iters, double *outc,
+                          double *outsse, int *outstatus)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || !prog || !off || !pcount || !c0 || !rowset || !metric || nprog <= 0 || kmax <= 0 ||
+        iters < 0 || !outc || !outsse || !outstatus)
+        return -1.0f;
+    if (kmax > FT_KMAX) return -1.0f;
+    return fit_run(c, prog, off, pcount, nprog, kmax, c0, rowset, metric, 0, iters, outc, outsse,
+                   outstatus);
+}
+
+int rc_fit_kernel_info(int kmax, int *threads, int *smem_bytes, int *max_constants)
+{
+    if (!threads || !smem_bytes || !max_constants) return -1;
+    if (kmax <= 0 || kmax > FT_KMAX) return -1;
+    *threads = 64;
+    *smem_bytes = (kmax + 2 <= 16 ? 16 : 32) * XS * 4;
+    *max_constants = FT_KMAX;
+    return 0;
+}
+
+int rc_fit_profile(void *ctx, int enable)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c) return -1;
+    if (!c->fit) c->fit = new RcFitState();
+    const int prev = c->fit->profile;
+    c->fit->profile = enable ? 1 : 0;
+    for (int q = 0; q < 4; ++q) {
+        c->fit->t_pass[q] = 0.0;
+        c->fit->n_pass[q] = 0;
+    }
+    return prev;
+}
+
+int rc_fit_report(void *ctx, RcFitReport *out, int *per_cand, int nprog)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || !out || !c->fit) return -1;
+    RcFitState *f = c->fit;
+    out->asm_ms = f->t_pass[0];
+    out->trial_ms = f->t_pass[1];
+    out->exact_ms = f->t_pass[2];
+    out->ctrl_ms = f->t_pass[3];
+    out->asm_launches = f->n_pass[0];
+    out->trial_launches = f->n_pass[1];
+    out->exact_launches = f->n_pass[2];
+    out->ctrl_launches = f->n_pass[3];
+    if (per_cand) {
+        if (!f->last_state || nprog != f->last_nprog) return -1;
+        std::vector<FtState> h((size_t)nprog);
+        if (hipMemcpy(h.data(), f->last_state, sizeof(FtState) * (size_t)nprog,
+                      hipMemcpyDeviceToHost) != hipSuccess)
+            return -1;
+        for (int i = 0; i < nprog; ++i) {
+            per_cand[3 * i + 0] = h[i].n_asm;
+            per_cand[3 * i + 1] = h[i].n_trial;
+            per_cand[3 * i + 2] = h[i].n_exact;
+        }
+    }
+    return 0;
+}
+
+int rc_set_fit_mode(void *ctx, int mode)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || (mode != 0 && mode != 1)) return -1;
+    if (!c->fit) c->fit = new RcFitState();
+    const int prev = c->fit->mode;
+    c->fit->mode = mode;
+    return prev;
+}
+
+int rc_fit_jacobian(void *ctx, const unsigned int *prog, const unsigned int *off, const int *pcount,
+                    int nprog, int kmax, const double *cvals, int cand, int rows, float *out_pred,
+                    float *out_jac, float *out_gx)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || !prog || !off || !pcount || !cvals || cand < 0 || cand >= nprog || rows <= 0 ||
+        kmax <= 0 || kmax > FT_KMAX || !out_pred || !out_jac || !out_gx)
+        return -1;
+    RcRows *R = &c->sets[0];
+    if (R->v > 8) return -1;
+    rows = std::min(rows, R->n);
+    FtProg p = translate_fit(prog + off[cand], (int)(off[cand + 1] - off[cand]), pcount[cand], R->v,
+                             true, true);
+    if (p.pre_status >= 0) return -1;
+    if (p.max_stack + 1 > BIG_S || p.max_adj + 2 > BIG_AS || p.tape_l + 1 > BIG_TAPE ||
+        p.tape_r + 1 > BIG_TAPE)
+        return -1;
+    std::vector<uint32_t> code = p.fwd;
+    FtCand d;
+    memset(&d, 0, sizeof(d));
+    d.fwd_off = 0;
+    d.rev_off = (uint32_t)code.size();
+    code.insert(code.end(), p.rev.begin(), p.rev.end());
+    d.m = (int)p.fwd.size();
+    d.nrev = (int)p.rev.size();
+    d.k = pcount[cand];
+    FtRows rd;
+    rd.cols = R->cols;
+    rd.cols64 = R->cols64;
+    rd.y64 = R->y64;
+    rd.w64 = R->w64;
+    rd.n = R->n;
+    rd.v = R->v;
+    std::vector<float> cp(FT_KMAX, 0.0f);
+    for (int j = 0; j < d.k; ++j) cp[j] = (float)cvals[(size_t)cand * kmax + j];
+    uint32_t *dc = nullptr;
+    FtCand *dd = nullptr;
+    FtRows *dr = nullptr;
+    float *dcp = nullptr, *dp = nullptr, *dj = nullptr, *dg = nullptr;
+    int rc = -1;
+    if (hipMalloc((void **)&dc, code.size() * 4) == hipSuccess &&
+        hipMalloc((void **)&dd, sizeof(FtCand)) == hipSuccess &&
+        hipMalloc((void **)&dr, sizeof(FtRows)) == hipSuccess &&
+        hipMalloc((void **)&dcp, FT_KMAX * 4) == hipSuccess &&
+        hipMalloc((void **)&dp, (size_t)rows * 4) == hipSuccess &&
+        hipMalloc((void **)&dj, (size_t)rows * kmax * 4) == hipSuccess &&
+        hipMalloc((void **)&dg, (size_t)rows * std::max(R->v, 1) * 4) == hipSuccess) {
+        (void)hipMemcpy(dc, code.data(), code.size() * 4, hipMemcpyHostToDevice);
+        (void)hipMemcpy(dd, &d, sizeof(FtCand), hipMemcpyHostToDevice);
+        (void)hipMemcpy(dr, &rd, sizeof(FtRows), hipMemcpyHostToDevice);
+        (void)hipMemcpy(dcp, cp.data(), FT_KMAX * 4, hipMemcpyHostToDevice);
+        (void)hipMemset(dj, 0, (size_t)rows * kmax * 4);
+        ft_probe_kernel<<<(rows + 63) / 64, 64, 0, c->s_fit>>>(dd, dr, dc, dcp, 0, rows, dp, dj, kmax, dg);
+        if (hipStreamSynchronize(c->s_fit) == hipSuccess && hipGetLastError() == hipSuccess) {
+            (void)hipMemcpy(out_pred, dp, (size_t)rows * 4, hipMemcpyDeviceToHost);
+            (void)hipMemcpy(out_jac, dj, (size_t)rows * kmax * 4, hipMemcpyDeviceToHost);
+            (void)hipMemcpy(out_gx, dg, (size_t)rows * R->v * 4, hipMemcpyDeviceToHost);
+            rc = 0;
+        }
+    }
+    (void)hipGetLastError();
+    if (dc) (void)hipFree(dc);
+    if (dd) (void)hipFree(dd);
+    if (dr) (void)hipFree(dr);
+    if (dcp) (void)hipFree(dcp);
+    if (dp) (void)hipFree(dp);
+    if (dj) (void)hipFree(dj);
+    if (dg) (void)hipFree(dg);
+    return rc;
+}
+
+float rc_implicit_score_batch(void *, const unsigned int *, const unsigned int *, const int *, int,
+                              int, const double *, int, double *, int *)
+{
+    return -1.0f;
+}
+
+float rc_implicit_fit_batch(void *, const unsigned int *, const unsigned int *, const int *, int, int,
+                            const double *, int, double *, double *, int *)
+{
+    return -1.0f;
+}
+
+} /* extern "C" */
+

@@ -1,0 +1,261 @@
+/*
+ * Device pieces of the batched LM / IRLS fit (included by rc_fit.hip).
+ *
+ * Program form.  The ABI's post-order records [op, kid0, kid1, slot]
+ * (fit.rs:1101-1196) are translated on the host into two wave-uniform
+ * instruction streams per candidate:
+ *   forward  one word per node: op | flags << 8 | arg << 16, executed as a
+ *            stack machine; flags ask the node to push the local partial
+ *            derivatives its on-path children need onto the tape;
+ *   reverse  one word per node on a path to a constant (or an input, for the
+ *            input gradient), in reverse post-order: the adjoint sweep of
+ *            ft_backward (eval.cu:1236-1317) becomes an adjoint *stack*
+ *            (a tree node has one parent, so the next on-path node in
+ *            reverse post-order always consumes the top of the stack) and the
+ *            tape of partials is a LIFO in the same order.
+ * Value stack, adjoint stack and tape live in VGPRs as LLVM vector values
+ * indexed by wave-uniform positions (s_set_gpr_idx), so neither sweep touches
+ * scratch or LDS.
+ */
+#ifndef RC_FIT_DEV_H
+#define RC_FIT_DEV_H
+
+#include "rc_device.h"
+
+#define FT_KMAX 20
+#define FT_MAX_NODES 96
+#define FT_TRI_MAX 253 /* (20 + 2)(20 + 3) / 2: [J | r_hi | r_lo] lower triangle */
+
+/* Forward flags. */
+#define FF_D0 1u /* push the partial w.r.t. the left (or only) child */
+#define FF_D1 2u /* push the partial w.r.t. the right child */
+
+/* Reverse kinds. */
+#define RK_CONST 0u
+#define RK_VAR 1u
+#define RK_ADD 2u
+#define RK_SUB 3u
+#define RK_NEG 4u
+#define RK_UNARY 5u
+#define RK_BINARY 6u
+#define RF_LEFT 1u
+#define RF_RIGHT 2u
+
+/* Per-candidate descriptor (host-built). */
+struct FtCand {
+    uint32_t fwd_off;
+    uint32_t rev_off;
+    int m;       /* forward words */
+    int nrev;    /* reverse words */
+    int k;       /* constants */
+    int rowset;
+    int metric;
+    int item0;   /* first work item */
+    int nitems;
+};
+
+/* One resident row set as the kernels see it. */
+struct FtRows {
+    const float *cols;
+    const double *cols64;
+    const double *y64;
+    const double *w64;
+    int n;
+    int v;
+};
+
+/* One work item: a tile of one candidate's rows. */
+struct FtItem {
+    int cand;
+    int row0;
+    int rows;
+    int pad;
+};
+
+typedef float fv8 __attribute__((ext_vector_type(8)));
+typedef float fv16 __attribute__((ext_vector_type(16)));
+typedef float fv32 __attribute__((ext_vector_type(32)));
+typedef double dv8 __attribute__((ext_vector_type(8)));
+typedef double dv16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+/* ---- value semantics ----------------------------------------------------- */
+
+/* f32: the reference device forward (ft_forward, eval.cu:1173-1233). */
+__device__ __forceinline__ float ft_unary(unsigned op, float a)
+{
+    switch (op) {
+        case OP_SIN: return sinf(a);
+        case OP_COS: return cosf(a);
+        case OP_TAN: { float t = tanf(a); return isfinite(t) ? t : RC_F32_NAN; }
+        case OP_EXP: return (a > 700.0f) ? __builtin_inff() : expf(a);
+        case OP_LOG: return (a > 0.0f) ? logf(a) : RC_F32_NAN;
+        case OP_SQRT: return (a >= 0.0f) ? sqrtf(a) : RC_F32_NAN;
+        case OP_ABS: return fabsf(a);
+        case OP_TANH: return tanhf(a);
+        case OP_SIG: return rc_logistic_f(a);
+        case OP_SQUARE: return a * a;
+        case OP_CUBE: return a * a * a;
+        case OP_NEG: return -a;
+        case OP_GAUSS: return expf(-a * a);
+        case OP_SINH: return sinhf(a);
+        case OP_COSH: return coshf(a);
+        case OP_ERF: return (a == 0.0f) ? 0.0f : erff(a);
+        case OP_ERFC: return erfcf(a);
+        default: return RC_F32_NAN;
+    }
+}
+
+__device__ __forceinline__ float ft_binary(unsigned op, float a, float b)
+{
+    switch (op) {
+        case OP_ADD: return a + b;
+        case OP_SUB: return a - b;
+        case OP_MUL: return a * b;
+        case OP_DIV: return (fabsf(b) < 1e-12f) ? RC_F32_NAN : a / b;
+        case OP_POW: return (a < 0.0f && fabsf(b - roundf(b)) > 1e-9f) ? RC_F32_NAN : powf(a, b);
+        default: return RC_F32_NAN;
+    }
+}
+
+/* f64: the CPU fitter's value_of (fit.rs:1211-1365). */
+__device__ __forceinline__ double ft_unary(unsigned op, double a)
+{
+    switch (op) {
+        case OP_SIN: return sin(a);
+        case OP_COS: return cos(a);
+        case OP_TAN: return tan(a);
+        case OP_EXP: return (a > 700.0) ? __builtin_inf() : exp(a);
+        case OP_LOG: return log(a);
+        case OP_SQRT: return sqrt(a);
+        case OP_ABS: return fabs(a);
+        case OP_TANH: return tanh(a);
+        case OP_SIG: return 1.0 / (1.0 + exp(-a));
+        case OP_SQUARE: return a * a;
+        case OP_CUBE: return a * a * a;
+        case OP_NEG: return -a;
+        case OP_GAUSS: return exp(-a * a);
+        case OP_SINH: return sinh(a);
+        case OP_COSH: return cosh(a);
+        case OP_ERF: return erf(a);
+        case OP_ERFC: return erfc(a);
+        default: return RC_F64_NAN;
+    }
+}
+
+__device__ __forceinline__ double ft_binary(unsigned op, double a, double b)
+{
+    switch (op) {
+        case OP_ADD: return a + b;
+        case OP_SUB: return a - b;
+        case OP_MUL: return a * b;
+        case OP_DIV: return (fabs(b) < 1e-12) ? RC_F64_NAN : a / b;
+        case OP_POW: return pow(a, b);
+        default: return RC_F64_NAN;
+    }
+}
+
+__device__ __forceinline__ bool ft_is_binary(unsigned op)
+{
+    return op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV || op == OP_POW;
+}
+
+/* Local partial of a unary operator at operand x with value `out`; the same
+   derivative table as ft_backward (eval.cu:1261-1315). */
+template <typename T>
+__device__ __forceinline__ T ft_unary_partial(unsigned op, T x, T out)
+{
+    switch (op) {
+        case OP_NEG: return (T)-1;
+        case OP_EXP: return out;
+        case OP_LOG: return (T)1 / x;
+        case OP_ABS: return x >= (T)0 ? (T)1 : (T)-1;
+        case OP_SQRT: return (T)1 / ((T)2 * out);
+        case OP_SIN: return cos(x);
+        case OP_COS: return -sin(x);
+        case OP_TAN: { T c = cos(x); return (T)1 / (c * c); }
+        case OP_SINH: return cosh(x);
+        case OP_COSH: return sinh(x);
+        case OP_TANH: return (T)1 - out * out;
+        case OP_SIG: return out * ((T)1 - out);
+        case OP_GAUSS: return (T)-2 * x * out;
+        case OP_SQUARE: return (T)2 * x;
+        case OP_CUBE: return (T)3 * x * x;
+        case OP_ERF: return (T)1.1283791670955126 * exp(-x * x);
+        case OP_ERFC: return (T)-1.1283791670955126 * exp(-x * x);
+        default: return (T)0;
+    }
+}
+
+/* ---- the solve ----------------------------------------------------------- */
+
+/* ln(1e-12): the smallest identified fraction of parameter space. */
+#define FT_LN_DET_FLOOR (-27.631021115928547)
+
+/* Damped, Jacobi-equilibrated Cholesky solve of (A~ + lambda diag + 1e-12)
+   delta~ = -D^-1/2 J^T r with the scale-free ln det gate on the undamped
+   system (ft_solve, eval.cu:1874-1949).  jtj is the packed lower triangle
+   (i*(i+1)/2 + j).  Returns 0 ok, 4 rank-deficient but solved, 1 no finite
+   step.  Serial: run by one lane. */
+__device__ int ft_solve(const double *jtj, const double *jtr, double lambda, int p, double *delta,
+                        double *L)
+{
+    double dscale[FT_KMAX];
+    for (int i = 0; i < p; ++i) {
+        const double aii = jtj[i * (i + 1) / 2 + i];
+        if (!(aii > 0.0) || !isfinite(aii)) return 1;
+        dscale[i] = sqrt(aii);
+    }
+    int rank_deficient = 0;
+    double sum_ln_L = 0.0;
+    int ok = 1;
+    for (int i = 0; i < p && ok; ++i) {
+        for (int j = 0; j <= i; ++j) {
+            double s = jtj[i * (i + 1) / 2 + j] / (dscale[i] * dscale[j]);
+            if (!isfinite(s)) return 1;
+            for (int q = 0; q < j; ++q) s -= L[i * FT_KMAX + q] * L[j * FT_KMAX + q];
+            if (i == j) {
+                if (!(s > 0.0) || !isfinite(s)) { ok = 0; break; }
+                L[i * FT_KMAX + i] = sqrt(s);
+                sum_ln_L += 0.5 * log(s);
+            } else {
+                L[i * FT_KMAX + j] = s / L[j * FT_KMAX + j];
+            }
+        }
+    }
+    if (!ok) {
+        rank_deficient = 1;
+    } else {
+        const double ln_det = 2.0 * sum_ln_L;
+        if (!isfinite(ln_det) || ln_det < FT_LN_DET_FLOOR) rank_deficient = 1;
+    }
+    for (int i = 0; i < p; ++i) {
+        for (int j = 0; j <= i; ++j) {
+            double s = jtj[i * (i + 1) / 2 + j] / (dscale[i] * dscale[j]);
+            if (i == j) s = s * (1.0 + lambda) + 1e-12;
+            for (int q = 0; q < j; ++q) s -= L[i * FT_KMAX + q] * L[j * FT_KMAX + q];
+            if (i == j) {
+                if (!(s > 1e-18) || !isfinite(s)) return 1;
+                L[i * FT_KMAX + i] = sqrt(s);
+            } else {
+                L[i * FT_KMAX + j] = s / L[j * FT_KMAX + j];
+            }
+        }
+    }
+    double rhs[FT_KMAX];
+    for (int i = 0; i < p; ++i) rhs[i] = -jtr[i] / dscale[i];
+    for (int i = 0; i < p; ++i) {
+        double s = rhs[i];
+        for (int q = 0; q < i; ++q) s -= L[i * FT_KMAX + q] * rhs[q];
+        rhs[i] = s / L[i * FT_KMAX + i];
+    }
+    for (int i = p - 1; i >= 0; --i) {
+        double s = rhs[i];
+        for (int q = i + 1; q < p; ++q) s -= L[q * FT_KMAX + i] * delta[q];
+        delta[i] = s / L[i * FT_KMAX + i];
+    }
+    for (int i = 0; i < p; ++i) delta[i] /= dscale[i];
+    return rank_deficient ? 4 : 0;
+}
+
+#endif

@@ -1,0 +1,129 @@
+/*
+ * Host-side context shared by the translation units of librcsrhip.so.
+ *
+ * One context owns one resident dataset (row set 0, uploaded by rc_create and
+ * rc_set_fit_data) plus optional extra row sets for heterogeneous batches, the
+ * evaluator's program arena, the fit region and the streams.  It mirrors the
+ * role of the reference's RcCtx (eval.cu:138-269) with an MI355X layout:
+ * column-major f32 rows for the mixed-precision kernels, column-major f64
+ * twins for the exact passes, everything resident in HBM for the context's
+ * life.
+ */
+#ifndef RC_INTERNAL_H
+#define RC_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/rcsr_hip.h"
+
+#define RC_MAX_STACK 64
+#define RC_MAX_SETS 16
+
+/* One resident row set.  cols[var * n + row]. */
+struct RcRows {
+    int n = 0;
+    int v = 0;
+    float *cols = nullptr;
+    float *y = nullptr;
+    double *cols64 = nullptr;
+    double *y64 = nullptr;
+    double *w64 = nullptr;
+    double ybar = 0.0;
+    double syy = 0.0;
+};
+
+struct RcFitState; /* rc_fit.hip */
+
+struct RcCtx {
+    RcRows sets[RC_MAX_SETS];
+    int nsets = 1;
+    float *d_tsign = nullptr;
+    double *d_tsign64 = nullptr;
+    int has_tsign = 0;
+
+    /* Evaluator arena: two program slots (translated internal streams) so an
+       upload may overlap the previous batch, a persistent output and the
+       per-tile partial sums. */
+    uint32_t *arena_code[2] = {nullptr, nullptr};
+    uint32_t *arena_meta[2] = {nullptr, nullptr}; /* per program: code offset */
+    size_t arena_code_words = 0;
+    size_t arena_meta_words = 0;
+    int arena_slot = 0;
+    int nprog = 0;
+    int max_stack = 0;
+    /* Programs grouped by peak stack depth (<=8, <=16, <=32, <=64): class k
+       occupies [cls_start[k], cls_start[k+1]) of the program list stored
+       after the code offsets in the meta arena. */
+    int cls_start[5] = {0, 0, 0, 0, 0};
+    double *d_part = nullptr; /* per (program, tile) partial sums */
+    size_t part_cap = 0;
+    int *d_part_bad = nullptr;
+    size_t part_bad_cap = 0;
+    float *d_out = nullptr;
+    size_t out_cap = 0;
+    double *d_score = nullptr;
+    int *d_status = nullptr;
+    size_t score_cap = 0;
+    uint32_t *h_stage = nullptr; /* pinned upload staging */
+    size_t h_stage_bytes = 0;
+    float *h_out = nullptr;      /* pinned download staging */
+    size_t h_out_bytes = 0;
+
+    /* Fixed-width program pool (rc_pool_*). */
+    uint32_t *pool_raw[2] = {nullptr, nullptr};
+    int pool_cap = 0;
+    int pool_width = 0;
+    std::vector<uint32_t> pool_host[2];
+    unsigned long long *d_hash = nullptr;
+
+    hipStream_t s_eval = nullptr;
+    hipStream_t s_fit = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipEvent_t ev_done = nullptr;
+
+    /* Pending evaluator batch (submit/collect split). */
+    int pending = 0;
+    int pending_loss = 0;
+    int pending_nprog = 0;
+
+    RcStats stats{};
+    double batch_start_ms = 0.0;
+
+    int async_copy = 1;
+    char sync_reason[96] = {0};
+    int warm = 0;
+    unsigned long long alloc_calls = 0;
+    unsigned long long steady_alloc_calls = 0;
+    unsigned long long overlapped_uploads = 0;
+    unsigned long long batches = 0;
+    unsigned long long fs_alloc_calls = 0;
+    unsigned long long fs_growths = 0;
+    unsigned long long fs_steady_alloc_calls = 0;
+    unsigned long long fs_batches = 0;
+
+    RcFitState *fit = nullptr;
+};
+
+/* Host helpers (rc_ctx.hip). */
+double rc_now_ms();
+int rc_dev_alloc(RcCtx *c, void **p, size_t bytes);
+void rc_dev_free(RcCtx *c, void *p);
+int rc_grow(RcCtx *c, void **p, size_t *cap, size_t need, size_t elem);
+int rc_grow_pinned(RcCtx *c, void **p, size_t *cap, size_t need);
+int rc_rows_upload(RcRows *r, const float *cols, const float *y, int n, int v);
+void rc_rows_free(RcRows *r);
+void rc_fit_state_free(RcCtx *c);
+
+/* Translate the ABI's postfix programs into the evaluator's internal stream
+   (rc_eval.hip).  Returns the batch's peak stack depth. */
+int rc_translate_programs(const uint32_t *prog, const uint32_t *off, int nprog, int nvars,
+                          std::vector<uint32_t> &code, std::vector<uint32_t> &meta,
+                          int cls_start[5]);
+
+#define RC_CHECK(x) do { if ((x) != hipSuccess) return -1; } while (0)
+
+#endif

@@ -1,0 +1,445 @@
+"""Host-side mirror of the reference's device-arm interface.
+
+Names, argument meaning and error behaviour follow the reference's Rust host
+layer so callers and tests read alike:
+
+  compile_program / Population      gpu.rs:198-218, 972-1068
+  gpu_fit_plan / GpuFitReject       fit.rs:958-1056
+  GpuEvaluator                      gpu.rs:1182-1969 (new, set_fit_data,
+                                    eval_batch, score_batch, predict,
+                                    fit_batch, GpuFitOutput gpu.rs:828)
+  fit_status_usable                 fit/gpu.rs:103
+  BatchedGpu / fit_batch_verified   fit/gpu.rs:113-317, 456-654
+
+The program builders run in C++ inside librcsrhip.so (rcsr_host.h); this
+module only marshals arrays.  The exact CPU arm that the verifier falls back
+to is the caller's (the reference keeps it unchanged); it is passed in as an
+object with `sse`, `metric_score` and `fit_metric` methods.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from enum import Enum
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import lib as _L
+
+HOST_MAX_STACK = 56          # gpu.rs:134
+GPU_FIT_MAX_CONSTANTS = 20   # fit.rs:851
+GPU_FIT_MAX_NODES = 96       # fit.rs:854
+
+METRIC_SQUARED, METRIC_LOG_SQUASHED, METRIC_ABSOLUTE = 0, 1, 4
+
+FIT_MODE_MIXED, FIT_MODE_FP64 = 0, 1
+
+
+class CompileError(Exception):
+    """gpu.rs:142 CompileError: Unsupported(op) or StackOverflow{peak, limit}."""
+
+    def __init__(self, kind: str, peak: int = 0):
+        super().__init__(kind if kind != "stack_overflow" else
+                         f"postfix program needs {peak} stack slots, the kernel allows "
+                         f"{HOST_MAX_STACK}")
+        self.kind = kind
+        self.peak = peak
+
+
+def _tree_arrays(tree) -> Tuple[np.ndarray, np.ndarray]:
+    ops, vals = tree.preorder()
+    return np.ascontiguousarray(ops, np.int32), np.ascontiguousarray(vals, np.float64)
+
+
+def compile_program(tree) -> np.ndarray:
+    """Postfix words, HALT-terminated (gpu.rs:198)."""
+    L = _L.load()
+    ops, vals = _tree_arrays(tree)
+    cap = 16 * len(ops) + 64
+    out = np.zeros(cap, np.uint32)
+    peak = C.c_int(0)
+    n = L.rch_compile_program(_L.ptr(ops, _L.i32p), _L.ptr(vals, _L.f64p), len(ops),
+                              _L.ptr(out, _L.u32p), cap, C.byref(peak))
+    if n == -1:
+        raise CompileError("unsupported")
+    if n == -2:
+        raise CompileError("stack_overflow", peak.value)
+    if n < 0:
+        raise CompileError("malformed")
+    return out[:n].copy()
+
+
+class Population:
+    """Packed postfix programs plus offsets (gpu.rs:972-1068)."""
+
+    def __init__(self):
+        self.prog: List[np.ndarray] = []
+        self.offsets = [0]
+        self.unsupported = 0
+
+    def add(self, tree) -> bool:
+        try:
+            w = compile_program(tree)
+        except CompileError:
+            self.unsupported += 1
+            return False
+        self.prog.append(w)
+        self.offsets.append(self.offsets[-1] + len(w))
+        return True
+
+    def __len__(self) -> int:
+        return len(self.prog)
+
+    def arrays(self) -> Tuple[np.ndarray, np.ndarray]:
+        prog = np.concatenate(self.prog).astype(np.uint32) if self.prog else np.zeros(1, np.uint32)
+        return np.ascontiguousarray(prog), np.asarray(self.offsets, np.uint32)
+
+
+class GpuFitReject(Enum):
+    """fit.rs:958-983."""
+    NO_CONSTANTS = "no_constants"
+    TOO_MANY_CONSTANTS = "too_many_constants"
+    TOO_MANY_NODES = "too_many_nodes"
+    SHARED_CONSTANTS = "shared_constants"
+    UNSUPPORTED_OP = "unsupported_op"
+
+
+_REJECT = {-1: GpuFitReject.NO_CONSTANTS, -2: GpuFitReject.TOO_MANY_CONSTANTS,
+           -3: GpuFitReject.TOO_MANY_NODES, -5: GpuFitReject.UNSUPPORTED_OP}
+
+
+@dataclass
+class GpuFitPlan:
+    """fit.rs:929-950: 4-word post-order records and the start constants."""
+    words: np.ndarray
+    nconst: int
+    c0: np.ndarray
+    nodes: int
+
+
+def gpu_fit_plan_reason(tree):
+    """Returns a GpuFitPlan or a GpuFitReject (fit.rs:985)."""
+    L = _L.load()
+    ops, vals = _tree_arrays(tree)
+    cap = 4 * (len(ops) + 1)
+    words = np.zeros(cap, np.uint32)
+    c0 = np.zeros(GPU_FIT_MAX_CONSTANTS + 1)
+    nodes = C.c_int(0)
+    k = L.rch_fit_plan(_L.ptr(ops, _L.i32p), _L.ptr(vals, _L.f64p), len(ops),
+                       _L.ptr(words, _L.u32p), cap, _L.ptr(c0, _L.f64p), len(c0), C.byref(nodes))
+    if k < 0:
+        if k in _REJECT:
+            return _REJECT[k]
+        raise ValueError("malformed tree")
+    return GpuFitPlan(words[:4 * nodes.value].copy(), k, c0[:k].copy(), nodes.value)
+
+
+def gpu_fit_plan(tree) -> Optional[GpuFitPlan]:
+    r = gpu_fit_plan_reason(tree)
+    return r if isinstance(r, GpuFitPlan) else None
+
+
+def fit_status_usable(status: int) -> bool:
+    """fit/gpu.rs:103: statuses 0 and 4-8 carry a finite answer."""
+    return status in (0, 4, 5, 6, 7, 8)
+
+
+@dataclass
+class GpuFitOutput:
+    """gpu.rs:828."""
+    c: np.ndarray
+    sse: float
+    status: int
+
+
+def pack_plans(plans: Sequence[GpuFitPlan], kmax: Optional[int] = None):
+    kmax = kmax or max(1, max(p.nconst for p in plans))
+    words = np.concatenate([p.words for p in plans]).astype(np.uint32)
+    off = np.zeros(len(plans) + 1, np.uint32)
+    off[1:] = np.cumsum([len(p.words) for p in plans])
+    pcount = np.asarray([p.nconst for p in plans], np.int32)
+    c0 = np.zeros((len(plans), kmax))
+    for i, p in enumerate(plans):
+        c0[i, :p.nconst] = p.c0
+    return np.ascontiguousarray(words), off, pcount, np.ascontiguousarray(c0), kmax
+
+
+class GpuEvaluator:
+    """Owns one device context with a resident dataset (gpu.rs:1182).
+
+    `cols` is column-major f32 (cols[var * n + row]), like the reference.
+    """
+
+    def __init__(self, cols, n: int, v: int, y):
+        self._L = _L.load()
+        self.n, self.v = int(n), int(v)
+        self._cols = np.ascontiguousarray(cols, np.float32)
+        self._y = np.ascontiguousarray(y, np.float32)
+        self.ctx = self._L.rc_create(_L.ptr(self._cols, _L.f32p), self.n, self.v,
+                                     _L.ptr(self._y, _L.f32p))
+        if not self.ctx:
+            raise RuntimeError("rc_create failed (no usable device?)")
+
+    @classmethod
+    def new(cls, cols, n, v, y) -> Optional["GpuEvaluator"]:
+        try:
+            return cls(cols, n, v, y)
+        except RuntimeError:
+            return None
+
+    def close(self):
+        if self.ctx:
+            self._L.rc_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- data -------------------------------------------------------------
+    def set_fit_data(self, cols64, y64, w=None) -> bool:
+        c64 = np.ascontiguousarray(cols64, np.float64)
+        y64 = np.ascontiguousarray(y64, np.float64)
+        ww = None if w is None or len(w) == 0 else np.ascontiguousarray(w, np.float64)
+        return self._L.rc_set_fit_data(self.ctx, _L.ptr(c64, _L.f64p), _L.ptr(y64, _L.f64p),
+                                       _L.ptr(ww, _L.f64p)) == 0
+
+    def add_rows(self, cols, n, v, y, cols64=None, y64=None, w=None) -> int:
+        cols = np.ascontiguousarray(cols, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        c64 = None if cols64 is None else np.ascontiguousarray(cols64, np.float64)
+        yy = None if y64 is None else np.ascontiguousarray(y64, np.float64)
+        ww = None if w is None else np.ascontiguousarray(w, np.float64)
+        return self._L.rc_add_rows(self.ctx, _L.ptr(cols, _L.f32p), int(n), int(v),
+                                   _L.ptr(y, _L.f32p), _L.ptr(c64, _L.f64p), _L.ptr(yy, _L.f64p),
+                                   _L.ptr(ww, _L.f64p))
+
+    def set_fit_mode(self, mode: int) -> int:
+        return self._L.rc_set_fit_mode(self.ctx, int(mode))
+
+    # -- evaluation -------------------------------------------------------
+    def eval_batch(self, trees, loss: int = 0):
+        """Per-tree f32 loss sums (None for a tree the kernel cannot run)."""
+        pop = Population()
+        ok = [pop.add(t) for t in trees]
+        if len(pop) == 0:
+            return [None] * len(trees), 0.0
+        prog, off = pop.arrays()
+        out = np.zeros(len(pop), np.float32)
+        self._L.rc_stats_begin(self.ctx)
+        if self._L.rc_set_programs(self.ctx, _L.ptr(prog, _L.u32p), _L.ptr(off, _L.u32p),
+                                   len(pop)) != 0:
+            return [None] * len(trees), -1.0
+        ms = self._L.rc_eval(self.ctx, _L.ptr(out, _L.f32p), int(loss))
+        it = iter(out.tolist())
+        return [next(it) if k else None for k in ok], ms
+
+    def eval_words(self, prog: np.ndarray, off: np.ndarray, loss: int = 0):
+        prog = np.ascontiguousarray(prog, np.uint32)
+        off = np.ascontiguousarray(off, np.uint32)
+        nprog = len(off) - 1
+        out = np.zeros(nprog, np.float32)
+        if self._L.rc_set_programs(self.ctx, _L.ptr(prog, _L.u32p), _L.ptr(off, _L.u32p),
+                                   nprog) != 0:
+            return None, -1.0
+        ms = self._L.rc_eval(self.ctx, _L.ptr(out, _L.f32p), int(loss))
+        return out, ms
+
+    def score_words(self, prog, off, kind: int, pcount=None):
+        prog = np.ascontiguousarray(prog, np.uint32)
+        off = np.ascontiguousarray(off, np.uint32)
+        nprog = len(off) - 1
+        out = np.zeros(nprog)
+        st = np.zeros(nprog, np.int32)
+        pc = None if pcount is None else np.ascontiguousarray(pcount, np.int32)
+        ms = self._L.rc_score_batch(self.ctx, _L.ptr(prog, _L.u32p), _L.ptr(off, _L.u32p), nprog,
+                                    int(kind), _L.ptr(pc, _L.i32p), _L.ptr(out, _L.f64p),
+                                    _L.ptr(st, _L.i32p))
+        return out, st, ms
+
+    def score_batch(self, trees, kind: int):
+        pop = Population()
+        ok = [pop.add(t) for t in trees]
+        prog, off = pop.arrays()
+        out, st, ms = self.score_words(prog, off, kind)
+        it = iter(zip(out.tolist(), st.tolist()))
+        return [next(it) if k else None for k in ok], ms
+
+    def predict_words(self, prog, off) -> Optional[np.ndarray]:
+        prog = np.ascontiguousarray(prog, np.uint32)
+        off = np.ascontiguousarray(off, np.uint32)
+        nprog = len(off) - 1
+        out = np.zeros((nprog, self.n), np.float32)
+        rc = self._L.rc_predict(self.ctx, _L.ptr(prog, _L.u32p), _L.ptr(off, _L.u32p), nprog,
+                                _L.ptr(out, _L.f32p))
+        return None if rc != 0 else out
+
+    def predict(self, trees) -> Optional[np.ndarray]:
+        pop = Population()
+        for t in trees:
+            if not pop.add(t):
+                raise CompileError("unsupported")
+        prog, off = pop.arrays()
+        return self.predict_words(prog, off)
+
+    # -- fit --------------------------------------------------------------
+    def fit_batch(self, plans: Sequence[GpuFitPlan], iters: int, metric: int = 0,
+                  kmax: Optional[int] = None, c0=None):
+        """gpu.rs:1909: returns ([GpuFitOutput], ms); ms < 0 discards all."""
+        words, off, pcount, c0p, kmax = pack_plans(plans, kmax)
+        if c0 is not None:
+            c0p = np.ascontiguousarray(c0, np.float64)
+        n = len(plans)
+        outc = np.zeros((n, kmax))
+        outsse = np.zeros(n)
+        outst = np.zeros(n, np.int32)
+        gauge = np.full(2 * n, -1, np.int32)
+        ms = self._L.rc_fit_batch(self.ctx, _L.ptr(words, _L.u32p), _L.ptr(off, _L.u32p),
+                                  _L.ptr(pcount, _L.i32p), n, kmax, _L.ptr(c0p, _L.f64p),
+                                  _L.ptr(gauge, _L.i32p), int(iters), int(metric),
+                                  _L.ptr(outc, _L.f64p), _L.ptr(outsse, _L.f64p),
+                                  _L.ptr(outst, _L.i32p))
+        outs = [GpuFitOutput(outc[i, :plans[i].nconst].copy(), float(outsse[i]), int(outst[i]))
+                for i in range(n)]
+        return outs, ms
+
+    def fit_batch_hetero(self, plans, rowset, metric, iters: int, kmax=None):
+        words, off, pcount, c0p, kmax = pack_plans(plans, kmax)
+        n = len(plans)
+        rs = np.ascontiguousarray(rowset, np.int32)
+        mt = np.ascontiguousarray(metric, np.int32)
+        outc = np.zeros((n, kmax))
+        outsse = np.zeros(n)
+        outst = np.zeros(n, np.int32)
+        ms = self._L.rc_fit_batch_hetero(self.ctx, _L.ptr(words, _L.u32p), _L.ptr(off, _L.u32p),
+                                         _L.ptr(pcount, _L.i32p), n, kmax, _L.ptr(c0p, _L.f64p),
+                                         _L.ptr(rs, _L.i32p), _L.ptr(mt, _L.i32p), int(iters),
+                                         _L.ptr(outc, _L.f64p), _L.ptr(outsse, _L.f64p),
+                                         _L.ptr(outst, _L.i32p))
+        outs = [GpuFitOutput(outc[i, :plans[i].nconst].copy(), float(outsse[i]), int(outst[i]))
+                for i in range(n)]
+        return outs, ms
+
+    def fit_jacobian(self, plans, c, cand: int, rows: int):
+        words, off, pcount, c0p, kmax = pack_plans(plans)
+        cc = np.ascontiguousarray(c, np.float64).reshape(len(plans), kmax)
+        pred = np.zeros(rows, np.float32)
+        jac = np.zeros((rows, kmax), np.float32)
+        gx = np.zeros((rows, max(self.v, 1)), np.float32)
+        rc = self._L.rc_fit_jacobian(self.ctx, _L.ptr(words, _L.u32p), _L.ptr(off, _L.u32p),
+                                     _L.ptr(pcount, _L.i32p), len(plans), kmax,
+                                     _L.ptr(cc, _L.f64p), int(cand), int(rows),
+                                     _L.ptr(pred, _L.f32p), _L.ptr(jac, _L.f32p),
+                                     _L.ptr(gx, _L.f32p))
+        return (pred, jac, gx) if rc == 0 else None
+
+    def implicit_score_batch(self, plans, metric: int = 0):
+        words, off, pcount, c0p, kmax = pack_plans(plans)
+        n = len(plans)
+        out = np.zeros(n)
+        st = np.zeros(n, np.int32)
+        ms = self._L.rc_implicit_score_batch(self.ctx, _L.ptr(words, _L.u32p), _L.ptr(off, _L.u32p),
+                                             _L.ptr(pcount, _L.i32p), n, kmax,
+                                             _L.ptr(c0p, _L.f64p), int(metric),
+                                             _L.ptr(out, _L.f64p), _L.ptr(st, _L.i32p))
+        return out, st, ms
+
+    def implicit_fit_batch(self, plans, iters: int):
+        words, off, pcount, c0p, kmax = pack_plans(plans)
+        n = len(plans)
+        outc = np.zeros((n, kmax))
+        outsse = np.zeros(n)
+        outst = np.zeros(n, np.int32)
+        ms = self._L.rc_implicit_fit_batch(self.ctx, _L.ptr(words, _L.u32p), _L.ptr(off, _L.u32p),
+                                           _L.ptr(pcount, _L.i32p), n, kmax, _L.ptr(c0p, _L.f64p),
+                                           int(iters), _L.ptr(outc, _L.f64p),
+                                           _L.ptr(outsse, _L.f64p), _L.ptr(outst, _L.i32p))
+        outs = [GpuFitOutput(outc[i, :plans[i].nconst].copy(), float(outsse[i]), int(outst[i]))
+                for i in range(n)]
+        return outs, ms
+
+    def stats(self) -> "_L.RcStats":
+        s = _L.RcStats()
+        self._L.rc_last_stats(self.ctx, C.byref(s))
+        return s
+
+    def arena_report(self) -> "_L.RcArenaReport":
+        r = _L.RcArenaReport()
+        self._L.rc_arena_report(self.ctx, C.byref(r))
+        return r
+
+    @staticmethod
+    def device_name() -> str:
+        name = _L.load().rc_device_name()
+        return name.decode() if name else "unknown"
+
+
+class BatchStatus(Enum):
+    """fit/gpu.rs:61-93."""
+    GPU = "gpu"
+    UNSUPPORTED = "unsupported"
+    NON_FINITE = "non_finite"
+    WORSE = "worse"
+
+
+class BatchedGpu:
+    """The BatchedFitter backed by the device (fit/gpu.rs:456-654).
+
+    fit_batch returns, per tree, the proposed constants or None (declined,
+    non-usable status, or a failed launch: the caller's exact arm refits).
+    """
+
+    def __init__(self, evaluator: GpuEvaluator):
+        self.ev = evaluator
+
+    def fit_batch_metric(self, trees, iters: int, metric: int) -> List[Optional[np.ndarray]]:
+        plans = [gpu_fit_plan(t) for t in trees]
+        idx = [i for i, p in enumerate(plans) if p is not None]
+        out: List[Optional[np.ndarray]] = [None] * len(trees)
+        if not idx:
+            return out
+        res, ms = self.ev.fit_batch([plans[i] for i in idx], iters, metric)
+        if ms < 0:
+            return out
+        for j, i in enumerate(idx):
+            if fit_status_usable(res[j].status) and np.all(np.isfinite(res[j].c)):
+                out[i] = res[j].c
+        return out
+
+
+def fit_batch_verified(backend: BatchedGpu, trees, x, y, w, iters: int, metric: int, exact):
+    """fit/gpu.rs:176-317: accept a proposal only if finite and no worse in
+    exact f64 (SSE for squared, the metric otherwise); else the exact arm.
+
+    Returns (trees, scores, statuses) with the trees carrying the kept
+    constants.  `exact` supplies sse / metric_score / fit_metric.
+    """
+    proposals = backend.fit_batch_metric(trees, iters, metric)
+    out_trees, scores, states = [], [], []
+    for tree, c in zip(trees, proposals):
+        if c is None:
+            s, cc = exact.fit_metric(tree, x, y, w, iters, metric)
+            out_trees.append(tree.with_constants(cc))
+            scores.append(s)
+            states.append(BatchStatus.UNSUPPORTED)
+            continue
+        trial = tree.with_constants(c)
+        if metric == METRIC_SQUARED:
+            start = exact.sse(tree, x, y, w)
+            cand = exact.sse(trial, x, y, w)
+        else:
+            start = exact.metric_score(tree, x, y, w, metric)
+            cand = exact.metric_score(trial, x, y, w, metric)
+        if not np.isfinite(cand) or not np.isfinite(start) or cand > start:
+            s, cc = exact.fit_metric(tree, x, y, w, iters, metric)
+            out_trees.append(tree.with_constants(cc))
+            scores.append(s)
+            states.append(BatchStatus.WORSE)
+        else:
+            out_trees.append(trial)
+            scores.append(cand)
+            states.append(BatchStatus.GPU)
+    return out_trees, scores, states
