@@ -250,6 +250,93 @@ __device__ __forceinline__ void load_x(const double *cols, int n, int v, int row
     for (int q = 0; q < 8; ++q) xv[q] = (q < v) ? cols[(size_t)q * (size_t)n + (size_t)row] : 0.0;
 }
 
+template <typename T>
+__device__ __noinline__ T ft_unary_call(unsigned op, T a) { return ft_unary(op, a); }
+template <typename T>
+__device__ __noinline__ T ft_binary_call(unsigned op, T a, T b) { return ft_binary(op, a, b); }
+
+/* Value-only forward of R rows per lane (rows xrow[r]): the evaluator's
+   interpreter shape.  The top of stack is cached, the values below it live
+   in st[r][d] (VGPRs indexed by the wave-uniform depth while R * S is small),
+   the common operators are inlined and the rare ones are out-of-line calls, so
+   the scalar dispatch of each instruction is amortised over R rows. */
+template <typename T, int S, int R>
+__device__ __forceinline__ void ft_value_rows(const uint32_t *__restrict__ fc, int m,
+                                              const T *__restrict__ cst,
+                                              const T *__restrict__ cols, int n,
+                                              const int *xrow, T *res)
+{
+    T tos[R];
+    T st[R][S];
+    int sp = 0, depth = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) tos[r] = (T)0;
+    for (int i = 0; i < m; ++i) {
+        const uint32_t w = fc[i];
+        const uint32_t op = w & 0xffu;
+        if (op == OP_VAR || op == OP_CONST) {
+            if (depth > 0) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) st[r][sp] = tos[r];
+                ++sp;
+            }
+            ++depth;
+            if (op == OP_VAR) {
+                const T *col = cols + (size_t)(w >> 16) * (size_t)n;
+#pragma unroll
+                for (int r = 0; r < R; ++r) tos[r] = col[xrow[r]];
+            } else {
+                const T c = cst[w >> 16];
+#pragma unroll
+                for (int r = 0; r < R; ++r) tos[r] = c;
+            }
+        } else if (op == OP_MUL) {
+            --sp; --depth;
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = st[r][sp] * tos[r];
+        } else if (op == OP_ADD) {
+            --sp; --depth;
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = st[r][sp] + tos[r];
+        } else if (op == OP_SUB) {
+            --sp; --depth;
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = st[r][sp] - tos[r];
+        } else if (op == OP_DIV) {
+            --sp; --depth;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const T b = tos[r];
+                tos[r] = (fabs(b) < (T)1e-12) ? (T)RC_F64_NAN : st[r][sp] / b;
+            }
+        } else if (op == OP_POW) {
+            --sp; --depth;
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = ft_binary_call<T>(op, st[r][sp], tos[r]);
+        } else if (op == OP_EXP) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = (tos[r] > (T)700) ? (T)__builtin_inf() : exp(tos[r]);
+        } else if (op == OP_SIN) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = sin(tos[r]);
+        } else if (op == OP_COS) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = cos(tos[r]);
+        } else if (op == OP_NEG) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = -tos[r];
+        } else if (op == OP_SQUARE) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = tos[r] * tos[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) tos[r] = ft_unary_call<T>(op, tos[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) res[r] = tos[r];
+}
+
 /* ---- assembly: forward + adjoint + MFMA Gram ------------------------------ */
 
 /* POLISH: the exact polish's assembly.  The residual comes from an f64
@@ -400,6 +487,10 @@ __global__ __launch_bounds__(64) void ft_trial_kernel(
     const uint32_t *__restrict__ code, const int *__restrict__ run, const T *__restrict__ cp,
     double *__restrict__ ploss)
 {
+    /* Rows per lane: 4 for f32 and 2 for f64 in the common class keep the
+       stack in VGPRs; the oversized class runs one row per lane. */
+    constexpr int R = BIG ? 1 : (sizeof(T) == 4 ? 4 : 2);
+    constexpr int S = BIG ? BIG_S : SMALL_S;
     const int li = blockIdx.x;
     if (li >= nlist) return;
     const int item = list[li];
@@ -407,30 +498,35 @@ __global__ __launch_bounds__(64) void ft_trial_kernel(
     const int cand = it.cand;
     if (!run[cand]) return;
     const FtCand d = cands[cand];
-    const FtRows R = rowsd[d.rowset];
+    const FtRows R_ = rowsd[d.rowset];
     const int lane = threadIdx.x;
     const T *cst = cp + (size_t)cand * FT_KMAX;
     const uint32_t *fc = code + d.fwd_off;
     const T *cols;
-    if (sizeof(T) == 4) cols = (const T *)(const void *)R.cols;
-    else cols = (const T *)(const void *)R.cols64;
+    if (sizeof(T) == 4) cols = (const T *)(const void *)R_.cols;
+    else cols = (const T *)(const void *)R_.cols64;
     double lsum = 0.0;
     const int row_end = it.row0 + it.rows;
-    for (int base = it.row0; base < row_end; base += 64) {
-        const int row = base + lane;
-        bool live = row < row_end;
-        const int rr = live ? row : it.row0;
-        const double wb = R.w64 ? R.w64[rr] : 1.0;
-        live = live && (wb > 0.0);
-        typename XVec<T>::type xv;
-        load_x(cols, R.n, R.v, rr, xv);
-        Tape2<T, BIG> tape;
-        const T pred = ft_forward<T, BIG, false>(fc, d.m, cst, xv, cols, R.n, rr, tape);
-        const double r = (double)pred - R.y64[rr];
-        if (!live) continue;
-        bool hard = !isfinite(pred) || !isfinite(r);
-        if (sizeof(T) == 4) hard = hard || fabs((double)pred) > 1e30;
-        lsum += hard ? 1e30 : wb * rc_row_rho(r, d.metric);
+    for (int base = it.row0; base < row_end; base += 64 * R) {
+        int xrow[R];
+        bool live[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = base + r * 64 + lane;
+            live[r] = row < row_end;
+            xrow[r] = live[r] ? row : it.row0;
+        }
+        T pred[R];
+        ft_value_rows<T, S, R>(fc, d.m, cst, cols, R_.n, xrow, pred);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const double wb = R_.w64 ? R_.w64[xrow[r]] : 1.0;
+            if (!live[r] || !(wb > 0.0)) continue;
+            const double rr = (double)pred[r] - R_.y64[xrow[r]];
+            bool hard = !isfinite(pred[r]) || !isfinite(rr);
+            if (sizeof(T) == 4) hard = hard || fabs((double)pred[r]) > 1e30;
+            lsum += hard ? 1e30 : wb * rc_row_rho(rr, d.metric);
+        }
     }
     const double ls = rc_wave_sum_f64(lsum);
     if (lane == 0) ploss[item] = ls;
@@ -1054,6 +1150,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         const int n = c->sets[d.rowset].n;
         d.item0 = (int)items.size();
         const bool big = prog_is_big(progs[i]);
+        const bool tbig = progs[i].max_stack > SMALL_S;
         const int cls = (d.k + 2 <= 16 ? 0 : 1) + (big ? 2 : 0);
         maxK = std::max(maxK, d.k + 2);
         for (int r0 = 0; r0 < n; r0 += TILE_ROWS) {
@@ -1063,7 +1160,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
             it.rows = std::min(TILE_ROWS, n - r0);
             it.pad = 0;
             list[cls].push_back((int)items.size());
-            tlist[big ? 1 : 0].push_back((int)items.size());
+            tlist[tbig ? 1 : 0].push_back((int)items.size());
             items.push_back(it);
         }
         d.nitems = (int)items.size() - d.item0;
