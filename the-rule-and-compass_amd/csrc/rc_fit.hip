@@ -29,6 +29,7 @@
  * a hipGraph and replayed.
  */
 #include "rc_fit_dev.h"
+#include "rc_fitvm_dev.h"
 #include "rc_internal.h"
 
 #include <math.h>
@@ -520,6 +521,244 @@ __global__ __launch_bounds__(64) void ft_trial_kernel(
         ft_value_rows<T, S, R>(fc, d.m, cst, cols, R_.n, xrow, pred);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
+            const double wb = R_.w64 ? R_.w64[xrow[r]] : 1.0;
+            if (!live[r] || !(wb > 0.0)) continue;
+            const double rr = (double)pred[r] - R_.y64[xrow[r]];
+            bool hard = !isfinite(pred[r]) || !isfinite(rr);
+            if (sizeof(T) == 4) hard = hard || fabs((double)pred[r]) > 1e30;
+            lsum += hard ? 1e30 : wb * rc_row_rho(rr, d.metric);
+        }
+    }
+    const double ls = rc_wave_sum_f64(lsum);
+    if (lane == 0) ploss[item] = ls;
+}
+
+/* ---- fit VM kernels: two rows per lane ----------------------------------- */
+
+constexpr int XS2 = 130; /* LDS column stride of the 128-row augmented block */
+
+/* Per-row residual bookkeeping shared by the assembly kernels. */
+struct RowPrep {
+    double r64, s64, wb;
+    bool live, ok;
+};
+
+__device__ __forceinline__ RowPrep vm_row_prep(bool live, double wb, float pred, double p64,
+                                               double y64, int metric, bool polish)
+{
+    RowPrep q;
+    q.live = live;
+    q.wb = wb;
+    q.r64 = p64 - y64;
+    q.ok = live && isfinite(p64) && (polish || fabsf(pred) <= 1e30f) && isfinite(q.r64);
+    q.s64 = sqrt(rc_row_weight(wb, q.r64, metric));
+    return q;
+}
+
+/* Assembly on the fit VM: the same per-row contract as ft_asm_kernel (the
+   augmented row [s*J | s*r_hi | s*r_lo] of every row into the Gram block,
+   the loss sum, bad-row and skipped-row flags), 128 rows per wave step. */
+template <int KC, bool POLISH>
+__global__ __launch_bounds__(64) void vm_asm_kernel(
+    const FtItem *__restrict__ items, const int *__restrict__ list, int nlist,
+    const FtCand *__restrict__ cands, const FtRows *__restrict__ rowsd,
+    const uint32_t *__restrict__ code, const int *__restrict__ run, const float *__restrict__ cp32,
+    const double *__restrict__ cp64, double *__restrict__ pgram, int tri_stride,
+    double *__restrict__ ploss, int *__restrict__ pbad, int *__restrict__ pskip)
+{
+    const int li = blockIdx.x;
+    if (li >= nlist) return;
+    const int item = list[li];
+    const FtItem it = items[item];
+    const int cand = it.cand;
+    if (!run[cand]) return;
+    const FtCand d = cands[cand];
+    const FtRows R = rowsd[d.rowset];
+    const int lane = threadIdx.x;
+    const int k = d.k;
+
+    __shared__ float X[KC * XS2];
+    for (int i = lane; i < KC * XS2; i += 64) X[i] = 0.0f;
+    __syncthreads();
+
+    const float *cst = cp32 + (size_t)cand * FT_KMAX;
+    const double *cst64 = cp64 + (size_t)cand * FT_KMAX;
+    const uint2 *fc = (const uint2 *)(code + d.fwd_off);
+    const uint2 *rc = (const uint2 *)(code + d.rev_off);
+    char *xl = (char *)(X + lane);
+    floatx4 acc00 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
+    double g00[4] = {0, 0, 0, 0}, g10[4] = {0, 0, 0, 0}, g11[4] = {0, 0, 0, 0};
+    double lsum = 0.0;
+    int bad = 0, skip = 0;
+    vm_u64x16 U = (vm_u64x16)(0), B = (vm_u64x16)(0), U_ = U, B_ = B;
+    vm_u64x8 XV = (vm_u64x8)(0), XV_ = XV;
+    const int row_end = it.row0 + it.rows;
+    int step = 0;
+    for (int base = it.row0; base < row_end; base += 128, ++step) {
+        const int r0 = base + lane, r1 = base + 64 + lane;
+        const bool in0 = r0 < row_end, in1 = r1 < row_end;
+        const int q0 = in0 ? r0 : it.row0, q1 = in1 ? r1 : it.row0;
+        const double wb0 = R.w64 ? R.w64[q0] : 1.0, wb1 = R.w64 ? R.w64[q1] : 1.0;
+#pragma unroll
+        for (int v = 0; v < VM_MAX_VARS; ++v)
+            if (v < R.v) {
+                const float *col = R.cols + (size_t)v * (size_t)R.n;
+                VmPack<vm_f2>::set(XV, XV_, v, (vm_f2){col[q0], col[q1]});
+            }
+        const vm_f2 pred = vm_forward<vm_f2, true>(fc, d.m, cst, XV, XV_, U, U_, B, B_);
+        double p0 = (double)pred.x, p1 = (double)pred.y;
+        if (POLISH) {
+            dv16 Ua = (dv16)(0.0), Ub = Ua, Ba = Ua, Bb = Ua;
+            dv8 Xa = (dv8)(0.0), Xb = Xa;
+#pragma unroll
+            for (int v = 0; v < VM_MAX_VARS; ++v)
+                if (v < R.v) {
+                    const double *col = R.cols64 + (size_t)v * (size_t)R.n;
+                    VmPack<vm_d2>::set(Xa, Xb, v, (vm_d2){col[q0], col[q1]});
+                }
+            const vm_d2 pp = vm_forward<vm_d2, false>(fc, d.m, cst64, Xa, Xb, Ua, Ub, Ba, Bb);
+            p0 = pp.x;
+            p1 = pp.y;
+        }
+        const RowPrep a0 = vm_row_prep(in0 && wb0 > 0.0, wb0, pred.x, p0, R.y64[q0], d.metric, POLISH);
+        const RowPrep a1 = vm_row_prep(in1 && wb1 > 0.0, wb1, pred.y, p1, R.y64[q1], d.metric, POLISH);
+        const vm_f2 seed = {a0.ok ? (float)a0.s64 : 0.0f, a1.ok ? (float)a1.s64 : 0.0f};
+        const vm_f2 fin = vm_reverse(rc, d.nrev, seed, cst, XV, U, B, xl);
+        const bool gf0 = fin.x == 0.0f, gf1 = fin.y == 0.0f;
+        const bool use0 = a0.ok && gf0, use1 = a1.ok && gf1;
+        if (POLISH) {
+            if ((a0.live && !a0.ok) || (a1.live && !a1.ok)) bad = 1;
+            skip += (a0.ok && !gf0 ? 1 : 0) + (a1.ok && !gf1 ? 1 : 0);
+            if (a0.ok) lsum += a0.wb * rc_row_rho(a0.r64, d.metric);
+            if (a1.ok) lsum += a1.wb * rc_row_rho(a1.r64, d.metric);
+        } else {
+            if ((a0.live && !use0) || (a1.live && !use1)) bad = 1;
+            if (use0) lsum += a0.wb * rc_row_rho(a0.r64, d.metric);
+            if (use1) lsum += a1.wb * rc_row_rho(a1.r64, d.metric);
+        }
+        if (__any((a0.ok && !gf0) || (a1.ok && !gf1))) {
+            for (int j = 0; j < k; ++j) {
+                if (!use0) X[j * XS2 + lane] = 0.0f;
+                if (!use1) X[j * XS2 + 64 + lane] = 0.0f;
+            }
+        }
+        const double rs0 = a0.s64 * a0.r64, rs1 = a1.s64 * a1.r64;
+        const float rh0 = (float)rs0, rh1 = (float)rs1;
+        X[k * XS2 + lane] = use0 ? rh0 : 0.0f;
+        X[(k + 1) * XS2 + lane] = use0 ? (float)(rs0 - (double)rh0) : 0.0f;
+        X[k * XS2 + 64 + lane] = use1 ? rh1 : 0.0f;
+        X[(k + 1) * XS2 + 64 + lane] = use1 ? (float)(rs1 - (double)rh1) : 0.0f;
+        __syncthreads();
+        const int col = lane & 15;
+        const int sub = lane >> 4;
+#pragma unroll 4
+        for (int q = 0; q < 32; ++q) {
+            const float b0 = X[col * XS2 + 4 * q + sub];
+            acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, b0, acc00, 0, 0, 0);
+            if (KC == 32) {
+                const float b1 = X[(16 + col) * XS2 + 4 * q + sub];
+                acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, b0, acc10, 0, 0, 0);
+                acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, b1, acc11, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+        if (step & 1) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                g00[r] += (double)acc00[r];
+                acc00[r] = 0.0f;
+                if (KC == 32) {
+                    g10[r] += (double)acc10[r];
+                    g11[r] += (double)acc11[r];
+                    acc10[r] = 0.0f;
+                    acc11[r] = 0.0f;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        g00[r] += (double)acc00[r];
+        if (KC == 32) {
+            g10[r] += (double)acc10[r];
+            g11[r] += (double)acc11[r];
+        }
+    }
+    const int K = k + 2;
+    double *pg = pgram + (size_t)item * (size_t)tri_stride;
+    const int j = lane & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = 4 * (lane >> 4) + r;
+        if (i >= j && i < K) pg[i * (i + 1) / 2 + j] = g00[r];
+        if (KC == 32) {
+            const int i1 = 16 + i;
+            if (i1 < K) pg[i1 * (i1 + 1) / 2 + j] = g10[r];
+            if (i >= j && i1 < K) pg[i1 * (i1 + 1) / 2 + 16 + j] = g11[r];
+        }
+    }
+    const double ls = rc_wave_sum_f64(lsum);
+    int sk = skip;
+    for (int off = 32; off > 0; off >>= 1) sk += __shfl_xor(sk, off, 64);
+    const int anybad = __any(bad) ? 1 : 0;
+    if (lane == 0) {
+        ploss[item] = ls;
+        pbad[item] = anybad;
+        pskip[item] = sk;
+    }
+}
+
+/* Trial / exact loss on the fit VM: forward only, 4 f32 rows or 2 f64 rows
+   per lane. */
+template <typename V>
+__global__ __launch_bounds__(64) void vm_trial_kernel(
+    const FtItem *__restrict__ items, const int *__restrict__ list, int nlist,
+    const FtCand *__restrict__ cands, const FtRows *__restrict__ rowsd,
+    const uint32_t *__restrict__ code, const int *__restrict__ run,
+    const typename VmScalar<V>::T *__restrict__ cp, double *__restrict__ ploss)
+{
+    typedef typename VmScalar<V>::T T;
+    constexpr int RR = VmScalar<V>::R;
+    const int li = blockIdx.x;
+    if (li >= nlist) return;
+    const int item = list[li];
+    const FtItem it = items[item];
+    const int cand = it.cand;
+    if (!run[cand]) return;
+    const FtCand d = cands[cand];
+    const FtRows R_ = rowsd[d.rowset];
+    const int lane = threadIdx.x;
+    const T *cst = cp + (size_t)cand * FT_KMAX;
+    const uint2 *fc = (const uint2 *)(code + d.fwd_off);
+    const T *cols;
+    if (sizeof(T) == 4) cols = (const T *)(const void *)R_.cols;
+    else cols = (const T *)(const void *)R_.cols64;
+    typedef VmPack<V> P;
+    typename P::F16 U0 = (typename P::F16)(0), U1 = U0, B0 = U0, B1 = U0;
+    typename P::F8 X0 = (typename P::F8)(0), X1 = X0;
+    double lsum = 0.0;
+    const int row_end = it.row0 + it.rows;
+    for (int base = it.row0; base < row_end; base += 64 * RR) {
+        int xrow[RR];
+        bool live[RR];
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            const int row = base + r * 64 + lane;
+            live[r] = row < row_end;
+            xrow[r] = live[r] ? row : it.row0;
+        }
+#pragma unroll
+        for (int v = 0; v < VM_MAX_VARS; ++v)
+            if (v < R_.v) {
+                const T *col = cols + (size_t)v * (size_t)R_.n;
+                V xv;
+#pragma unroll
+                for (int r = 0; r < RR; ++r) xv[r] = col[xrow[r]];
+                P::set(X0, X1, v, xv);
+            }
+        const V pred = vm_forward<V, false>(fc, d.m, cst, X0, X1, U0, U1, B0, B1);
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
             const double wb = R_.w64 ? R_.w64[xrow[r]] : 1.0;
             if (!live[r] || !(wb > 0.0)) continue;
             const double rr = (double)pred[r] - R_.y64[xrow[r]];
@@ -1024,6 +1263,9 @@ void rc_fit_state_free(RcCtx *c)
 }
 
 int rc_rows_ensure_f64(RcCtx *c, RcRows *r);
+extern "C" int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int col_bytes,
+                                 unsigned int *fwd, int cap_f, unsigned int *rev, int cap_r,
+                                 int *info);
 int rc_fit64_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const int *pcount, int nprog,
                  int kmax, const double *c0, const std::vector<int> &rowset,
                  const std::vector<int> &metric, const std::vector<int> &pre, int iters,
@@ -1129,9 +1371,14 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     std::vector<uint32_t> code;
     std::vector<FtItem> items;
     std::vector<int> pre((size_t)nprog, -1);
-    std::vector<int> list[4]; /* (KC16,small) (KC32,small) (KC16,big) (KC32,big) */
-    std::vector<int> tlist[2];
+    /* Assembly classes: fit VM (KC16, KC32), general interpreter with the
+       register stores (KC16, KC32) and with the oversized stores (KC16,
+       KC32).  Value-pass classes: fit VM, general small, general big. */
+    enum { NA = 6, NT = 3 };
+    std::vector<int> list[NA];
+    std::vector<int> tlist[NT];
     int maxK = 2;
+    std::vector<uint32_t> vf(2 * 4 * FT_MAX_NODES), vr(2 * 8 * FT_MAX_NODES);
     for (int i = 0; i < nprog; ++i) {
         FtCand &d = cands[i];
         memset(&d, 0, sizeof(d));
@@ -1141,17 +1388,35 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         /* A host-decided candidate keeps k = 0: its outputs are c0 verbatim. */
         d.k = (pre[i] >= 0) ? 0 : pcount[i];
         if (pre[i] >= 0) continue;
-        d.fwd_off = (uint32_t)code.size();
-        code.insert(code.end(), progs[i].fwd.begin(), progs[i].fwd.end());
-        d.rev_off = (uint32_t)code.size();
-        code.insert(code.end(), progs[i].rev.begin(), progs[i].rev.end());
-        d.m = (int)progs[i].fwd.size();
-        d.nrev = (int)progs[i].rev.size();
+        const int words = (int)(off[i + 1] - off[i]);
+        int info[6] = {0, 0, 0, 0, 0, 0};
+        const int vmrc = rch_fitvm_compile(prog + off[i], words, pcount[i], c->sets[d.rowset].v,
+                                           XS2 * 4, vf.data(), (int)vf.size(), vr.data(),
+                                           (int)vr.size(), info);
+        if (vmrc == 0) {
+            if (code.size() & 1) code.push_back(0);
+            d.vm = 1;
+            d.fwd_off = (uint32_t)code.size();
+            code.insert(code.end(), vf.begin(), vf.begin() + 2 * info[0]);
+            d.rev_off = (uint32_t)code.size();
+            code.insert(code.end(), vr.begin(), vr.begin() + 2 * info[1]);
+            d.m = info[0];
+            d.nrev = info[1];
+        } else {
+            d.fwd_off = (uint32_t)code.size();
+            code.insert(code.end(), progs[i].fwd.begin(), progs[i].fwd.end());
+            d.rev_off = (uint32_t)code.size();
+            code.insert(code.end(), progs[i].rev.begin(), progs[i].rev.end());
+            d.m = (int)progs[i].fwd.size();
+            d.nrev = (int)progs[i].rev.size();
+        }
         const int n = c->sets[d.rowset].n;
         d.item0 = (int)items.size();
         const bool big = prog_is_big(progs[i]);
         const bool tbig = progs[i].max_stack > SMALL_S;
-        const int cls = (d.k + 2 <= 16 ? 0 : 1) + (big ? 2 : 0);
+        const int kc = d.k + 2 <= 16 ? 0 : 1;
+        const int cls = d.vm ? kc : (big ? 4 : 2) + kc;
+        const int tcls = d.vm ? 0 : (tbig ? 2 : 1);
         maxK = std::max(maxK, d.k + 2);
         for (int r0 = 0; r0 < n; r0 += TILE_ROWS) {
             FtItem it;
@@ -1160,7 +1425,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
             it.rows = std::min(TILE_ROWS, n - r0);
             it.pad = 0;
             list[cls].push_back((int)items.size());
-            tlist[tbig ? 1 : 0].push_back((int)items.size());
+            tlist[tcls].push_back((int)items.size());
             items.push_back(it);
         }
         d.nitems = (int)items.size() - d.item0;
@@ -1169,16 +1434,13 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     const int nitems = (int)items.size();
     const int tri_stride = maxK * (maxK + 1) / 2;
     /* Longest items first within each class list: fewer stragglers. */
-    for (int q = 0; q < 4; ++q)
-        std::stable_sort(list[q].begin(), list[q].end(), [&](int a, int b) {
-            return (long long)cands[items[a].cand].m * items[a].rows >
-                   (long long)cands[items[b].cand].m * items[b].rows;
-        });
-    for (int q = 0; q < 2; ++q)
-        std::stable_sort(tlist[q].begin(), tlist[q].end(), [&](int a, int b) {
-            return (long long)cands[items[a].cand].m * items[a].rows >
-                   (long long)cands[items[b].cand].m * items[b].rows;
-        });
+    auto work = [&](int a) {
+        return (long long)(cands[items[a].cand].m + cands[items[a].cand].nrev) * items[a].rows;
+    };
+    for (int q = 0; q < NA; ++q)
+        std::stable_sort(list[q].begin(), list[q].end(), [&](int a, int b) { return work(a) > work(b); });
+    for (int q = 0; q < NT; ++q)
+        std::stable_sort(tlist[q].begin(), tlist[q].end(), [&](int a, int b) { return work(a) > work(b); });
     std::vector<FtRows> rowsd((size_t)c->nsets);
     for (int s = 0; s < c->nsets; ++s) {
         rowsd[s].cols = c->sets[s].cols;
@@ -1189,14 +1451,17 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         rowsd[s].v = c->sets[s].v;
     }
     std::vector<int> flat_lists;
-    int lofs[6];
-    for (int q = 0; q < 4; ++q) {
+    int lofs[NA + 1], tofs[NT + 1];
+    for (int q = 0; q < NA; ++q) {
         lofs[q] = (int)flat_lists.size();
         flat_lists.insert(flat_lists.end(), list[q].begin(), list[q].end());
     }
-    lofs[4] = (int)flat_lists.size();
-    for (int q = 0; q < 2; ++q) flat_lists.insert(flat_lists.end(), tlist[q].begin(), tlist[q].end());
-    lofs[5] = (int)flat_lists.size();
+    lofs[NA] = (int)flat_lists.size();
+    for (int q = 0; q < NT; ++q) {
+        tofs[q] = (int)flat_lists.size();
+        flat_lists.insert(flat_lists.end(), tlist[q].begin(), tlist[q].end());
+    }
+    tofs[NT] = (int)flat_lists.size();
     if (flat_lists.empty()) flat_lists.push_back(0);
 
     /* Region layout: inputs [code | cands | rows | items | lists | pre | c0]
@@ -1287,32 +1552,42 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     };
     auto asm_pass0 = [&](int polish) {
         if (polish) {
-            asm_launch(ft_asm_kernel<16, false, true>, 0);
-            asm_launch(ft_asm_kernel<32, false, true>, 1);
-            asm_launch(ft_asm_kernel<16, true, true>, 2);
-            asm_launch(ft_asm_kernel<32, true, true>, 3);
+            asm_launch(vm_asm_kernel<16, true>, 0);
+            asm_launch(vm_asm_kernel<32, true>, 1);
+            asm_launch(ft_asm_kernel<16, false, true>, 2);
+            asm_launch(ft_asm_kernel<32, false, true>, 3);
+            asm_launch(ft_asm_kernel<16, true, true>, 4);
+            asm_launch(ft_asm_kernel<32, true, true>, 5);
         } else {
-            asm_launch(ft_asm_kernel<16, false, false>, 0);
-            asm_launch(ft_asm_kernel<32, false, false>, 1);
-            asm_launch(ft_asm_kernel<16, true, false>, 2);
-            asm_launch(ft_asm_kernel<32, true, false>, 3);
+            asm_launch(vm_asm_kernel<16, false>, 0);
+            asm_launch(vm_asm_kernel<32, false>, 1);
+            asm_launch(ft_asm_kernel<16, false, false>, 2);
+            asm_launch(ft_asm_kernel<32, false, false>, 3);
+            asm_launch(ft_asm_kernel<16, true, false>, 4);
+            asm_launch(ft_asm_kernel<32, true, false>, 5);
         }
     };
     auto asm_pass = [&](int polish) { mark(0, [&]() { asm_pass0(polish); }); };
     auto trial_pass0 = [&](bool exact) {
-        for (int q = 0; q < 2; ++q) {
-            const int a = (q == 0) ? lofs[4] : lofs[4] + (int)tlist[0].size();
-            const int nl = (q == 0) ? (int)tlist[0].size() : (int)tlist[1].size();
+        for (int q = 0; q < NT; ++q) {
+            const int nl = tofs[q + 1] - tofs[q];
             if (nl <= 0) continue;
-            const int *L = d_lists + a;
-            if (!exact && q == 0)
-                ft_trial_kernel<float, false><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
-            else if (!exact)
-                ft_trial_kernel<float, true><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
-            else if (q == 0)
-                ft_trial_kernel<double, false><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
-            else
-                ft_trial_kernel<double, true><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
+            const int *L = d_lists + tofs[q];
+            if (!exact) {
+                if (q == 0)
+                    vm_trial_kernel<vm_f4><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
+                else if (q == 1)
+                    ft_trial_kernel<float, false><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
+                else
+                    ft_trial_kernel<float, true><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
+            } else {
+                if (q == 0)
+                    vm_trial_kernel<vm_d2><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
+                else if (q == 1)
+                    ft_trial_kernel<double, false><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
+                else
+                    ft_trial_kernel<double, true><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
+            }
         }
     };
 
@@ -1322,7 +1597,8 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     std::vector<uint64_t> key = {(uint64_t)nprog, (uint64_t)nitems, (uint64_t)iters,
                                  (uint64_t)any_irls, (uint64_t)kmax, (uint64_t)tri_stride,
                                  (uint64_t)dev_end, (uint64_t)(uintptr_t)D};
-    for (int q = 0; q < 6; ++q) key.push_back((uint64_t)lofs[q]);
+    for (int q = 0; q <= NA; ++q) key.push_back((uint64_t)lofs[q]);
+    for (int q = 0; q <= NT; ++q) key.push_back((uint64_t)tofs[q]);
     const char *nograph = getenv("RCSR_FIT_NO_GRAPH");
     const bool use_graph = !(nograph && nograph[0] == '1') && !f->profile;
     auto record = [&]() {

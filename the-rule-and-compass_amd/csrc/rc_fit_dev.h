@@ -52,6 +52,8 @@ struct FtCand {
     int metric;
     int item0;   /* first work item */
     int nitems;
+    int vm;      /* 1: fit VM code (word pairs, m / nrev count pairs) */
+    int pad[2];
 };
 
 /* One resident row set as the kernels see it. */

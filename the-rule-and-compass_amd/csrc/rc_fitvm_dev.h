@@ -1,0 +1,379 @@
+/*
+ * Device interpreter of the fit VM (encoding in rc_fitvm.h, compiler in
+ * rc_fitvm.cpp), included by rc_fit.hip.
+ *
+ * One lane carries several rows at once: a "pack" value holds 2 f32 rows
+ * (vm_f2), 4 f32 rows (vm_f4) or 2 f64 rows (vm_d2).  The register files
+ * store a slot as 64-bit elements of an LLVM vector, so one wave-uniform
+ * indexed move (s_set_gpr_idx) reads or writes the slot for two f32 rows at
+ * once, and the scalar cost of decoding and dispatching an instruction is
+ * shared by every row of the pack.
+ *
+ * Per node the arithmetic is ft_forward's (eval.cu:1173-1233) for f32 rows
+ * and value_of's (fit.rs:1211-1365) for f64 rows; the adjoint products are
+ * ft_backward's (eval.cu:1236-1317) with the tape partials of the general
+ * interpreter, and a zero adjoint contributes exactly 0.
+ */
+#ifndef RC_FITVM_DEV_H
+#define RC_FITVM_DEV_H
+
+#include "rc_fit_dev.h"
+#include "rc_fitvm.h"
+
+typedef unsigned long long vm_u64;
+typedef vm_u64 vm_u64x8 __attribute__((ext_vector_type(8)));
+typedef vm_u64 vm_u64x16 __attribute__((ext_vector_type(16)));
+typedef float vm_f2 __attribute__((ext_vector_type(2)));
+typedef float vm_f4 __attribute__((ext_vector_type(4)));
+typedef double vm_d2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ vm_f2 vm_unpack(vm_u64 v)
+{
+    vm_f2 r;
+    r.x = __uint_as_float((unsigned)v);
+    r.y = __uint_as_float((unsigned)(v >> 32));
+    return r;
+}
+
+__device__ __forceinline__ vm_u64 vm_pack(vm_f2 x)
+{
+    return (vm_u64)__float_as_uint(x.x) | ((vm_u64)__float_as_uint(x.y) << 32);
+}
+
+/* Register files are plain local vectors of 64-bit elements (a struct
+   holding them would be demoted to scratch): every file is a pair (f0, f1)
+   of such vectors.  A vm_f2 pack keeps rows 0-1 in f0 (f1 unused), a vm_f4
+   pack rows 0-1 in f0 and 2-3 in f1, a vm_d2 pack row 0 in f0 and row 1 in
+   f1.  Both halves of a slot share one index, so the compiler moves them
+   under one s_set_gpr_idx bracket. */
+template <typename V> struct VmPack;
+template <> struct VmPack<vm_f2> {
+    typedef float T;
+    typedef vm_u64x16 F16;
+    typedef vm_u64x8 F8;
+    enum { R = 2 };
+    template <typename F> static __device__ __forceinline__ vm_f2 get(const F &f0, const F &, unsigned i)
+    {
+        return vm_unpack(f0[i]);
+    }
+    template <typename F> static __device__ __forceinline__ void set(F &f0, F &, unsigned i, vm_f2 x)
+    {
+        f0[i] = vm_pack(x);
+    }
+};
+template <> struct VmPack<vm_f4> {
+    typedef float T;
+    typedef vm_u64x16 F16;
+    typedef vm_u64x8 F8;
+    enum { R = 4 };
+    template <typename F> static __device__ __forceinline__ vm_f4 get(const F &f0, const F &f1, unsigned i)
+    {
+        const vm_f2 a = vm_unpack(f0[i]), b = vm_unpack(f1[i]);
+        return (vm_f4){a.x, a.y, b.x, b.y};
+    }
+    template <typename F> static __device__ __forceinline__ void set(F &f0, F &f1, unsigned i, vm_f4 x)
+    {
+        f0[i] = vm_pack((vm_f2){x.x, x.y});
+        f1[i] = vm_pack((vm_f2){x.z, x.w});
+    }
+};
+template <> struct VmPack<vm_d2> {
+    typedef double T;
+    typedef dv16 F16;
+    typedef dv8 F8;
+    enum { R = 2 };
+    template <typename F> static __device__ __forceinline__ vm_d2 get(const F &f0, const F &f1, unsigned i)
+    {
+        return (vm_d2){f0[i], f1[i]};
+    }
+    template <typename F> static __device__ __forceinline__ void set(F &f0, F &f1, unsigned i, vm_d2 x)
+    {
+        f0[i] = x.x;
+        f1[i] = x.y;
+    }
+};
+
+template <typename V> struct VmScalar {
+    typedef typename VmPack<V>::T T;
+    enum { R = VmPack<V>::R };
+};
+
+/* Constant c of a y word: the byte offset is slot * 4 (f32 constants). */
+__device__ __forceinline__ float vm_const(const float *cst, uint32_t y)
+{
+    return *(const float *)((const char *)cst + VM_Y_COFS(y));
+}
+__device__ __forceinline__ double vm_const(const double *cst, uint32_t y)
+{
+    return *(const double *)((const char *)cst + 2u * VM_Y_COFS(y));
+}
+
+/* ---- per-component operator helpers ------------------------------------- */
+
+template <typename V> __device__ __forceinline__ V vm_bcast(typename VmScalar<V>::T c)
+{
+    return (V)(c);
+}
+
+/* ft_forward / value_of unary semantics on every row of a pack. */
+template <typename V> __device__ __noinline__ V vm_unary_call(unsigned op, V a)
+{
+    V r;
+#pragma unroll
+    for (int q = 0; q < VmScalar<V>::R; ++q) r[q] = ft_unary(op, a[q]);
+    return r;
+}
+
+template <typename V> __device__ __noinline__ V vm_partial_call(unsigned op, V a, V f)
+{
+    V r;
+#pragma unroll
+    for (int q = 0; q < VmScalar<V>::R; ++q) r[q] = ft_unary_partial<typename VmScalar<V>::T>(op, a[q], f[q]);
+    return r;
+}
+
+template <typename V> __device__ __noinline__ V vm_binary_call(unsigned op, V a, V b)
+{
+    V r;
+#pragma unroll
+    for (int q = 0; q < VmScalar<V>::R; ++q) r[q] = ft_binary(op, a[q], b[q]);
+    return r;
+}
+
+/* Local partials of a binary node (ft_forward's tape values). */
+template <typename V> __device__ __noinline__ void vm_binary_partials(unsigned op, V a, V b, V &d0, V &d1)
+{
+    typedef typename VmScalar<V>::T T;
+#pragma unroll
+    for (int q = 0; q < VmScalar<V>::R; ++q) {
+        T p0 = (T)0, p1 = (T)0;
+        if (op == OP_MUL) { p0 = b[q]; p1 = a[q]; }
+        else if (op == OP_DIV) { p0 = (T)1 / b[q]; p1 = -a[q] / (b[q] * b[q]); }
+        else if (op == OP_POW && a[q] > (T)0) {
+            p0 = b[q] * pow(a[q], b[q] - (T)1);
+            p1 = pow(a[q], b[q]) * log(a[q]);
+        }
+        d0[q] = p0;
+        d1[q] = p1;
+    }
+}
+
+__device__ __forceinline__ float vm_exp1(float a) { return (a > 700.0f) ? __builtin_inff() : expf(a); }
+__device__ __forceinline__ double vm_exp1(double a) { return (a > 700.0) ? __builtin_inf() : exp(a); }
+
+/* Unary value with the common operators inline. */
+template <typename V> __device__ __forceinline__ V vm_unary(uint32_t x, uint32_t y, V a)
+{
+    V r;
+    if (x & VMF_SIN) {
+#pragma unroll
+        for (int q = 0; q < VmScalar<V>::R; ++q) r[q] = sin(a[q]);
+    } else if (x & VMF_COS) {
+#pragma unroll
+        for (int q = 0; q < VmScalar<V>::R; ++q) r[q] = cos(a[q]);
+    } else if (x & VMF_EXP) {
+#pragma unroll
+        for (int q = 0; q < VmScalar<V>::R; ++q) r[q] = vm_exp1(a[q]);
+    } else {
+        r = vm_unary_call<V>(VMF_Y_OP(y), a);
+    }
+    return r;
+}
+
+template <typename V> __device__ __forceinline__ V vm_partial(uint32_t x, uint32_t y, V a, V f)
+{
+    V r;
+    if (x & VMF_SIN) {
+#pragma unroll
+        for (int q = 0; q < VmScalar<V>::R; ++q) r[q] = cos(a[q]);
+    } else if (x & VMF_COS) {
+#pragma unroll
+        for (int q = 0; q < VmScalar<V>::R; ++q) r[q] = -sin(a[q]);
+    } else if (x & VMF_EXP) {
+        r = f;
+    } else {
+        r = vm_partial_call<V>(VMF_Y_OP(y), a, f);
+    }
+    return r;
+}
+
+/* ---- forward sweep ----------------------------------------------------- */
+
+/* Runs the forward stream on one pack of rows.  With TAPE the partials the
+   reverse stream reads are left in U and B.  Every step writes U once (the
+   sink slot when it has nothing to keep), so no branch join merges two
+   versions of the file. */
+template <typename V, bool TAPE>
+__device__ __forceinline__ V vm_forward(const uint2 *__restrict__ fc, int nf,
+                                        const typename VmScalar<V>::T *__restrict__ cst,
+                                        const typename VmPack<V>::F8 &X0,
+                                        const typename VmPack<V>::F8 &X1,
+                                        typename VmPack<V>::F16 &U0, typename VmPack<V>::F16 &U1,
+                                        typename VmPack<V>::F16 &B0, typename VmPack<V>::F16 &B1)
+{
+    typedef VmPack<V> P;
+    V tos = vm_bcast<V>(0);
+    for (int i = 0; i < nf; ++i) {
+        const uint2 w = fc[i];
+        const uint32_t x = w.x, y = w.y;
+        V nt, uv = tos;
+        uint32_t us = VMF_Y_US(y);
+        if (x & VMF_UNMC) {
+            const V f = vm_unary<V>(x, y, tos);
+            if (TAPE && (x & VMF_NEED)) P::set(B0, B1, VMF_Y_BS(y), vm_partial<V>(x, y, tos, f));
+            uv = f;
+            nt = f * vm_const(cst, y);
+        } else if (x & VMF_VMC) {
+            nt = P::get(X0, X1, VMF_Y_XV(y)) * vm_const(cst, y);
+        } else if (x & VMF_ADD) {
+            nt = P::get(U0, U1, VMF_Y_UR(y)) + tos;
+        } else if (x & VMF_MULC) {
+            nt = tos * vm_const(cst, y);
+        } else if (x & VMF_PUSHV) {
+            nt = P::get(X0, X1, VMF_Y_XV(y));
+        } else if (x & VMF_UN) {
+            const V f = vm_unary<V>(x, y, tos);
+            if (TAPE && (x & VMF_NEED)) uv = vm_partial<V>(x, y, tos, f);
+            nt = f;
+        } else if (x & VMF_MULV) {
+            nt = tos * P::get(X0, X1, VMF_Y_XV(y));
+        } else if (x & VMF_ADDC) {
+            nt = tos + vm_const(cst, y);
+        } else if (x & VMF_PUSHC) {
+            nt = vm_bcast<V>(vm_const(cst, y));
+        } else {
+            V l, r;
+            if (x & VMF_RC) {
+                l = tos;
+                r = vm_bcast<V>(vm_const(cst, y));
+            } else if (x & VMF_RV) {
+                l = tos;
+                r = P::get(X0, X1, VMF_Y_XV(y));
+            } else {
+                l = P::get(U0, U1, VMF_Y_UR(y));
+                r = tos;
+            }
+            const unsigned op = VMF_Y_OP(y);
+            if (op == OP_ADD) nt = l + r;
+            else if (op == OP_SUB) nt = l - r;
+            else if (op == OP_MUL) nt = l * r;
+            else nt = vm_binary_call<V>(op, l, r);
+            if (TAPE && (x & (VMF_D0 | VMF_D1))) {
+                V d0, d1;
+                vm_binary_partials<V>(op, l, r, d0, d1);
+                uv = d0;
+                if (x & VMF_D1) P::set(B0, B1, VMF_Y_BS(y), d1);
+            }
+        }
+        P::set(U0, U1, us, uv);
+        tos = nt;
+    }
+    return tos;
+}
+
+/* ---- reverse sweep ----------------------------------------------------- */
+
+/* a * p with a zero adjoint contributing exactly 0 (ft_backward skips such
+   nodes). */
+__device__ __forceinline__ vm_f2 vm_zmul(vm_f2 a, vm_f2 p)
+{
+    vm_f2 r;
+    r.x = (a.x == 0.0f) ? 0.0f : a.x * p.x;
+    r.y = (a.y == 0.0f) ? 0.0f : a.y * p.y;
+    return r;
+}
+__device__ __forceinline__ vm_f2 vm_zmul(vm_f2 a, float c)
+{
+    vm_f2 r;
+    r.x = (a.x == 0.0f) ? 0.0f : a.x * c;
+    r.y = (a.y == 0.0f) ? 0.0f : a.y * c;
+    return r;
+}
+
+/* Gradient column write of two rows (row r and r + 64 of the block) plus the
+   finiteness accumulator (0 while every gradient is finite, NaN after). */
+__device__ __forceinline__ void vm_gwrite(char *xl, uint32_t x, vm_f2 g, vm_f2 &fin)
+{
+    float *p = (float *)(xl + (x >> VMR_GOFS_SHIFT));
+    p[0] = g.x;
+    p[64] = g.y;
+    fin.x += g.x * 0.0f;
+    fin.y += g.y * 0.0f;
+}
+
+/* Runs the reverse stream seeded with `seed`; gradient columns go to the LDS
+   block through xl (this lane's byte address of row 0, column 0).  Returns
+   the finiteness accumulator. */
+__device__ __forceinline__ vm_f2 vm_reverse(const uint2 *__restrict__ rc, int nr, vm_f2 seed,
+                                            const float *__restrict__ cst, const vm_u64x8 &X,
+                                            const vm_u64x16 &U, const vm_u64x16 &B, char *xl)
+{
+    typedef VmPack<vm_f2> P;
+    vm_u64x8 Ad = (vm_u64x8)(0), none;
+    vm_f2 a = seed, fin = {0.0f, 0.0f};
+    for (int i = 0; i < nr; ++i) {
+        const uint2 w = rc[i];
+        const uint32_t x = w.x, y = w.y;
+        if (x & VMR_UNMC_N) {
+            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, VMR_Y_UT(y))), fin);
+            a = vm_zmul(vm_zmul(a, vm_const(cst, y)), P::get(B, B, VMR_Y_BT(y)));
+        } else if (x & VMR_VMC) {
+            vm_gwrite(xl, x, vm_zmul(a, P::get(X, X, VMR_Y_XV(y))), fin);
+            a = P::get(Ad, Ad, VMR_Y_AR(y));
+        } else if (x & VMR_PUSH) {
+            P::set(Ad, none, VMR_Y_AS(y), a);
+        } else if (x & VMR_MULC_N) {
+            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, VMR_Y_UT(y))), fin);
+            a = vm_zmul(a, vm_const(cst, y));
+        } else if (x & VMR_UNMC_P) {
+            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, VMR_Y_UT(y))), fin);
+            a = P::get(Ad, Ad, VMR_Y_AR(y));
+        } else if (x & VMR_UN) {
+            a = vm_zmul(a, P::get(U, U, VMR_Y_UT(y)));
+        } else if (x & VMR_MULC_P) {
+            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, VMR_Y_UT(y))), fin);
+            a = P::get(Ad, Ad, VMR_Y_AR(y));
+        } else if (x & VMR_MULV) {
+            a = vm_zmul(a, P::get(X, X, VMR_Y_XV(y)));
+        } else if (x & VMR_ADDC_N) {
+            vm_gwrite(xl, x, a, fin);
+        } else if (x & VMR_ADDC_P) {
+            vm_gwrite(xl, x, a, fin);
+            a = P::get(Ad, Ad, VMR_Y_AR(y));
+        } else {
+            /* R_GEN: two word pairs. */
+            const uint32_t x2 = rc[i + 1].x;
+            ++i;
+            const unsigned op = VMR_Y_OP(y);
+            vm_f2 gl, gr;
+            if (x & VMR_UNARY) {
+                gl = vm_zmul(a, P::get(U, U, VMR_Y_UT(y)));
+                gr = gl;
+            } else if (op == OP_ADD) {
+                gl = a;
+                gr = a;
+            } else if (op == OP_SUB) {
+                gl = a;
+                gr = -a;
+            } else {
+                gl = vm_zmul(a, P::get(U, U, VMR_Y_UT(y)));
+                gr = vm_zmul(a, P::get(B, B, VMR_Y_BT(y)));
+            }
+            if (x & VMR_LCONST) vm_gwrite(xl, x, gl, fin);
+            if (x & VMR_RCONST) vm_gwrite(xl, x2, gr, fin);
+            if ((x & VMR_LNODE) && (x & VMR_RNODE)) {
+                P::set(Ad, none, VMR_Y_AS(y), gl);
+                a = gr;
+            } else if (x & VMR_LNODE) {
+                a = gl;
+            } else if (x & VMR_RNODE) {
+                a = gr;
+            } else {
+                a = P::get(Ad, Ad, VMR_Y_AR(y));
+            }
+        }
+    }
+    return fin;
+}
+
+#endif
