@@ -165,3 +165,25 @@ def test_implicit_fit_recovers_the_unit_circle():
     s, c = oracle.implicit_fit(f, X, None, 20)
     assert c[0] == pytest.approx(1.0, abs=1e-6)
     assert s < 1e-10
+
+
+def test_linear_lm_reaches_the_least_squares_optimum():
+    # gpu.rs:4388-4411 fixture: orthogonal sin((k+1)t) columns, p = 1..20.
+    # For a linear model the LM optimum is the least-squares solution, which
+    # numpy's SVD solver gives in closed form: a dropped damping term, a
+    # transposed Gram or a wrong step sign leaves the SSE above it.
+    n = 800
+    for p in range(1, 21):
+        t = 2 * np.pi * np.arange(n) / (n - 1)
+        x = np.stack([np.sin((k + 1) * t) for k in range(p)], 1)
+        noise = (((np.arange(n) * 37) % 11) - 5.0) * 2e-4
+        y = x @ np.asarray([1.0 + 0.31 * k for k in range(p)]) + noise
+        terms = [node(MUL, const(0.25 + 0.07 * k), var(k)) for k in range(p)]
+        tree = terms[-1]
+        for term in reversed(terms[:-1]):
+            tree = node(ADD, term, tree)
+        sse, c = oracle.fit_weighted(tree, x, y, None, 8)
+        coef, *_ = np.linalg.lstsq(x, y, rcond=None)
+        best = float(np.sum((x @ coef - y) ** 2))
+        assert abs(sse - best) <= 1e-9 * best, (p, sse, best)
+        assert np.max(np.abs(np.asarray(c) - coef)) <= 1e-6, p
