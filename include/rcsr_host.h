@@ -31,12 +31,15 @@ int rch_fit_plan(const int *ops, const double *vals, int nn, unsigned int *out_w
    register-slot VM (rc_fitvm.h), exported so its output can be checked on
    the CPU.  rec: words / 4 post-order fit records (the rch_fit_plan form);
    p constants; nvars inputs; col_bytes: bytes between gradient columns of
-   the assembly block.  fwd / rev receive (x, y) word pairs; info[0..5] =
-   forward pairs, reverse pairs, U slots, B slots, saved-adjoint slots, 0.
-   Returns 0, or -1 malformed records, -2 unsupported operator, -3 the
-   program does not fit the VM, -4 an output buffer is too small. */
+   the assembly block; flags: 1 inputs in U slots 0..nvars-1, 2 value only
+   (no tape, no reverse stream), 4 an 8-slot U file.  fwd / rev receive
+   (x, y) word pairs; info[0..5] = forward pairs, reverse pairs, U slots,
+   B slots, saved-adjoint slots, 0.  Returns 0, or -1 malformed records,
+   -2 unsupported operator, -3 the program does not fit the VM, -4 an
+   output buffer is too small. */
 int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int col_bytes,
-                      unsigned int *fwd, int cap_f, unsigned int *rev, int cap_r, int *info);
+                      unsigned int flags, unsigned int *fwd, int cap_f, unsigned int *rev,
+                      int cap_r, int *info);
 
 #ifdef __cplusplus
 }

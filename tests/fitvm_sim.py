@@ -25,7 +25,7 @@ def _defines():
         if m:
             out[m.group(1)] = 1 << int(m.group(2))
             continue
-        m = re.match(r"#define\s+(VM\w+)\s+(\d+)\s*$", line)
+        m = re.match(r"#define\s+(VM\w+)\s+(\d+)u?\s*(/\*.*)?$", line)
         if m:
             out[m.group(1)] = int(m.group(2))
     return out
@@ -146,12 +146,20 @@ class _File:
         self.w[i] = True
 
 
-def run(fwd, rev, c, X, col_bytes):
-    """Forward value and constant gradients of every row of X (rows x vars)."""
+def run(fwd, rev, c, X, col_bytes, flags=0):
+    """Forward value and constant gradients of every row of X (rows x vars).
+    flags: the compiler options the streams were built with (inputs in U,
+    an 8-slot U file)."""
     X = np.asarray(X, np.float64)
     rows = X.shape[0]
     c = np.asarray(c, np.float64)
-    U = _File("U", D["VM_U_SLOTS"], rows, D["VM_U_SINK"])
+    small = bool(flags & D["VMC_SMALL_U"])
+    usink = D["VM_U_SINK_SMALL"] if small else D["VM_U_SINK"]
+    U = _File("U", 8 if small else D["VM_U_SLOTS"], rows, usink)
+    in_u = bool(flags & D["VMC_INPUTS_IN_U"])
+    if in_u:
+        for q in range(X.shape[1]):
+            U.set(q, X[:, q])
     B = _File("B", D["VM_B_SLOTS"], rows, D["VM_B_SINK"])
     A = _File("A", D["VM_A_SLOTS"], rows, D["VM_A_SINK"])
     tos = np.zeros(rows)
@@ -162,12 +170,16 @@ def run(fwd, rev, c, X, col_bytes):
         return c[off // 4]
 
     def xin(i):
+        if in_u:
+            return U.get(i)
         return X[:, i]
 
     fw = np.asarray(fwd, np.uint32).reshape(-1, 2)
     for x, y in fw:
         x, y = int(x), int(y)
         us, ur, bs, xv, op = (y >> 8) & 15, (y >> 12) & 15, (y >> 16) & 15, (y >> 20) & 7, (y >> 23) & 31
+        if small:
+            ur &= 7
         uv = tos
         if x & D["VMF_UNMC"]:
             f = _unary(op, tos)
@@ -209,8 +221,12 @@ def run(fwd, rev, c, X, col_bytes):
                     B.set(bs, d1)
         else:
             raise AssertionError(f"forward word without a handler: {x:#x}")
+        if small:
+            us &= 7
+        if in_u and us < X.shape[1]:
+            raise SlotError(f"U: write into input slot {us}")
         U.v[us] = uv
-        if us != D["VM_U_SINK"]:
+        if us != usink:
             U.w[us] = True
         tos = nt
     pred = tos

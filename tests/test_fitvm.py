@@ -27,7 +27,10 @@ UNARY = [SIN, COS, TAN, EXP, LOG, SQRT, ABS, TANH, SIG, SQUARE, CUBE, NEG, GAUSS
 BINARY = [ADD, SUB, MUL, DIV, POW]
 
 
-def compile_tree(tree, nvars=3):
+INPUTS_IN_U, VALUE_ONLY, SMALL_U = 1, 2, 4
+
+
+def compile_tree(tree, nvars=3, flags=0):
     plan = host.gpu_fit_plan(tree)
     assert plan is not None
     L = lib.load()
@@ -36,16 +39,24 @@ def compile_tree(tree, nvars=3):
     rv = np.zeros(2 * 8 * 96, np.uint32)
     info = np.zeros(6, np.int32)
     rc = L.rch_fitvm_compile(lib.ptr(rec, lib.u32p), len(rec), plan.nconst, nvars, COL_BYTES,
-                             lib.ptr(fw, lib.u32p), len(fw), lib.ptr(rv, lib.u32p), len(rv),
+                             flags, lib.ptr(fw, lib.u32p), len(fw), lib.ptr(rv, lib.u32p), len(rv),
                              lib.ptr(info, lib.i32p))
     return rc, plan, fw[:2 * info[0]], rv[:2 * info[1]], info
 
 
-def check_tree(tree, X, rel=1e-9):
-    rc, plan, fw, rv, info = compile_tree(tree, X.shape[1])
+def check_tree(tree, X, rel=1e-9, flags=0):
+    rc, plan, fw, rv, info = compile_tree(tree, X.shape[1], flags)
     if rc != 0:
         return rc
-    pred, grads = fitvm_sim.run(fw, rv, plan.c0, X, COL_BYTES)
+    pred, grads = fitvm_sim.run(fw, rv, plan.c0, X, COL_BYTES, flags)
+    if flags & VALUE_ONLY:
+        assert len(rv) == 0 and not grads
+        want = oracle.predict(tree, X)
+        fin = np.isfinite(want) & np.isfinite(pred)
+        assert np.all(fin | (~np.isfinite(pred) & ~np.isfinite(want))), str(tree)
+        assert np.allclose(pred[fin], want[fin], rtol=rel, atol=rel), str(tree)
+        assert info[2] <= (7 if flags & SMALL_U else 15)
+        return 0
     want = oracle.predict(tree, X)
     # The evaluator semantics of oracle.predict turn some domain edges into
     # NaN where the fit's value_of gives +-inf: both non-finite agree.
@@ -152,15 +163,15 @@ def test_rejections():
     info = np.zeros(6, np.int32)
     buf = np.zeros(64, np.uint32)
     mod = np.asarray([0, 0, 0, 0, 26, 0, 0, 0], np.uint32)  # an operator without an adjoint
-    assert L.rch_fitvm_compile(lib.ptr(mod, lib.u32p), 8, 1, 3, COL_BYTES, lib.ptr(buf, lib.u32p),
+    assert L.rch_fitvm_compile(lib.ptr(mod, lib.u32p), 8, 1, 3, COL_BYTES, 0, lib.ptr(buf, lib.u32p),
                                64, lib.ptr(buf, lib.u32p), 64, lib.ptr(info, lib.i32p)) == -2
     bad = np.asarray([0, 0, 0, 0, 2, 0, 0, 0], np.uint32)  # ADD with one operand: malformed
-    assert L.rch_fitvm_compile(lib.ptr(bad, lib.u32p), 8, 1, 3, COL_BYTES, lib.ptr(buf, lib.u32p),
+    assert L.rch_fitvm_compile(lib.ptr(bad, lib.u32p), 8, 1, 3, COL_BYTES, 0, lib.ptr(buf, lib.u32p),
                                64, lib.ptr(buf, lib.u32p), 64, lib.ptr(info, lib.i32p)) == -1
     small = np.zeros(2, np.uint32)
     rc, plan, *_ = compile_tree(node(MUL, var(0), const(2.0)))
     rec = np.ascontiguousarray(plan.words, np.uint32)
-    assert L.rch_fitvm_compile(lib.ptr(rec, lib.u32p), len(rec), 1, 3, COL_BYTES,
+    assert L.rch_fitvm_compile(lib.ptr(rec, lib.u32p), len(rec), 1, 3, COL_BYTES, 0,
                                lib.ptr(small, lib.u32p), 0, lib.ptr(buf, lib.u32p), 64,
                                lib.ptr(info, lib.i32p)) == -4
 
@@ -173,14 +184,35 @@ def test_simulator_catches_a_planted_slot_error(X):
     assert rc == 0
     fw = fw.copy().reshape(-1, 2)
     add = [i for i in range(len(fw)) if fw[i, 0] & fitvm_sim.D["VMF_ADD"]][0]
-    good, _ = fitvm_sim.run(fw.ravel(), rv, plan.c0, X, COL_BYTES)
+    good, _ = fitvm_sim.run(fw.ravel(), rv, plan.c0, X, COL_BYTES, 0)
     ur = (int(fw[add, 1]) >> 12) & 15
     # another written slot: the value moves
     wrong = fw.copy()
     wrong[add, 1] = (int(wrong[add, 1]) & ~(15 << 12)) | (((ur + 1) % 3) << 12)
-    moved, _ = fitvm_sim.run(wrong.ravel(), rv, plan.c0, X, COL_BYTES)
+    moved, _ = fitvm_sim.run(wrong.ravel(), rv, plan.c0, X, COL_BYTES, 0)
     assert not np.allclose(moved, good)
     # a slot nothing wrote: refused
     wrong[add, 1] = (int(wrong[add, 1]) & ~(15 << 12)) | (14 << 12)
     with pytest.raises(fitvm_sim.SlotError):
-        fitvm_sim.run(wrong.ravel(), rv, plan.c0, X, COL_BYTES)
+        fitvm_sim.run(wrong.ravel(), rv, plan.c0, X, COL_BYTES, 0)
+
+
+@pytest.mark.parametrize("flags", [INPUTS_IN_U, INPUTS_IN_U | VALUE_ONLY | SMALL_U])
+def test_inputs_in_u_and_value_only_streams(flags, X):
+    # The value-pass stream keeps the inputs in U slots 0..v-1 and drops the
+    # tape; with an 8-slot file every slot index stays below 8.
+    rng = np.random.default_rng(99)
+    done = 0
+    while done < 150:
+        t = full_tree(rng, int(rng.integers(1, 6)))
+        if not t.constants() or host.gpu_fit_plan(t) is None:
+            continue
+        rc = check_tree(t, X, rel=1e-8, flags=flags)
+        assert rc in (0, -3), rc
+        done += 1
+    wl_rng = workloads.rng_for(3)
+    for k in list(range(1, 21)) * 2:
+        assert check_tree(workloads.structured_candidate(wl_rng, 3, k), X, rel=1e-8,
+                          flags=flags) == 0
+    for name, build in EDGE:
+        assert check_tree(build(), X, flags=flags) == 0, name

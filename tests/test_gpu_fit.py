@@ -391,3 +391,65 @@ def test_mixed_and_fp64_arms_agree_on_a_c3_slice():
     assert conv.sum() >= 5
     assert np.all(rm[conv] <= 1e-6)
     assert np.median(rf) <= 1e-9
+
+
+def test_c4_uneven_hetero_batch_matches_per_island_fits():
+    # Config 4 shape at test size: islands with log-uniform row counts, mixed
+    # squared / absolute metrics, Cauchy outliers, one launch per pass for the
+    # whole uneven batch (rc_fit_batch_hetero over resident row sets).
+    wl = workloads.config4(nisland=4, pop=24, nmin=512, nmax=24000, iters=8)
+    X, Y = wl.x, wl.y
+    sets = wl.rowsets
+    trees, plans, isl, met = [], [], [], []
+    for t, i, m in zip(wl.trees, wl.island_of, wl.metrics):
+        p = host.gpu_fit_plan(t)
+        if p is None:
+            continue
+        trees.append(t)
+        plans.append(p)
+        isl.append(int(i))
+        met.append(int(m))
+    x0, y0 = X[sets[0]], Y[sets[0]]
+    ev = _evaluator(x0, y0)
+    rowset = [0] * len(plans)
+    ids = {0: 0}
+    for s in range(1, len(sets)):
+        xs, ys = X[sets[s]], Y[sets[s]]
+        ids[s] = ev.add_rows(np.ascontiguousarray(xs.T, np.float32).reshape(-1), len(xs), xs.shape[1],
+                             ys.astype(np.float32), np.ascontiguousarray(xs.T).reshape(-1), ys, None)
+        assert ids[s] > 0
+    rowset = [ids[i] for i in isl]
+    outs, ms = ev.fit_batch_hetero(plans, rowset, met, 8, kmax=20)
+    ev.close()
+    assert ms >= 0
+    _say(f"hetero: {len(plans)} candidates over {len(sets)} islands in {ms:.1f} ms")
+    # Per (island, metric) single-set fits: identical bits.
+    for s in range(len(sets)):
+        xs, ys = X[sets[s]], Y[sets[s]]
+        for m in (0, 4):
+            sel = [i for i in range(len(plans)) if isl[i] == s and met[i] == m]
+            if not sel:
+                continue
+            ev1 = _evaluator(xs, ys)
+            single, _ = ev1.fit_batch([plans[i] for i in sel], 8, m, kmax=20)
+            ev1.close()
+            for o, i in zip(single, sel):
+                assert o.c.tobytes() == outs[i].c.tobytes(), (s, m, i)
+                assert o.status == outs[i].status
+    # Reported losses are the exact objectives of the returned constants, and
+    # no usable fit is worse than its start.
+    rels = []
+    for i, o in enumerate(outs):
+        if o.status not in (0, 4, 8):
+            continue
+        xs, ys = X[sets[isl[i]]], Y[sets[isl[i]]]
+        n = len(ys)
+        got = oracle.metric_score(trees[i].with_constants(o.c), xs, ys, None, met[i]) * n
+        start = oracle.metric_score(trees[i], xs, ys, None, met[i]) * n
+        assert o.sse == pytest.approx(got, rel=1e-9), i
+        assert got <= start * (1 + 1e-12) + 1e-12, i
+        if met[i] == 0:
+            cpu, _ = oracle.fit_metric(trees[i], xs, ys, None, 8, 0)
+            rels.append(abs(got - cpu) / max(cpu, 1e-300))
+    _say(f"squared candidates vs oracle: median rel {np.median(rels):.2e}")
+    assert np.median(rels) <= 1e-6

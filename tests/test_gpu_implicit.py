@@ -1,0 +1,178 @@
+"""GPU parity of the implicit-relation arm (rc_implicit_score_batch,
+rc_implicit_fit_batch) and of the adjoint input gradients (rc_fit_jacobian),
+through the C ABI.
+
+The reference scores and fits implicit relations on the CPU only
+(implicit.rs:59-411); the oracle restates it in f64 and is pinned by the
+exact unit-circle distance.  The device arm computes the same f64 distance
+f / ||grad_x f|| with an adjoint input-gradient sweep and the same
+central-difference LM, so scores are compared at relative 1e-9 and fitted
+losses at 1e-6.  The f32 adjoint gradients (the mixed-precision fit's
+Jacobian and the input gradient) are checked against central differences of
+the f64 oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from rcsr_hip import host, workloads
+from rcsr_hip.expr import (ADD, COS, EXP, LOG, MUL, SIN, SQUARE, SUB, const, node, var)
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _evaluator(X, w=None):
+    n, v = X.shape
+    ev = host.GpuEvaluator(np.ascontiguousarray(X.T, np.float32).reshape(-1), n, v,
+                           np.zeros(n, np.float32))
+    assert ev.set_fit_data(np.ascontiguousarray(X.T).reshape(-1), np.zeros(n), w)
+    return ev
+
+
+def _circle(n, seed=5):
+    wl = workloads.config5(n=n, ncand=24, iters=8)
+    return wl.x, wl.trees
+
+
+def _unit_circle_tree(r2):
+    return node(SUB, node(ADD, node(SQUARE, var(0)), node(SQUARE, var(1))), const(r2))
+
+
+def test_implicit_scores_match_the_oracle():
+    X, trees = _circle(4096)
+    trees = trees[:16] + [_unit_circle_tree(0.8),
+                          node(SUB, node(LOG, var(0)), const(0.1)),     # undefined for x1 <= 0
+                          node(ADD, node(MUL, const(0.0), var(0)), const(1.0))]  # zero gradient
+    plans = [host.gpu_fit_plan(t) for t in trees]
+    assert all(p is not None for p in plans)
+    ev = _evaluator(X)
+    for metric in (0, 1):
+        out, st, ms = ev.implicit_score_batch(plans, metric)
+        assert ms >= 0
+        for i, t in enumerate(trees):
+            want = oracle.implicit_score(t, X, None, metric)
+            _say(f"metric {metric} cand {i}: device {out[i]:.12e} status {st[i]} oracle {want:.12e}")
+            if math.isinf(want):
+                assert math.isinf(out[i]) and st[i] == 2, (i, out[i])
+            else:
+                assert st[i] == 0
+                assert out[i] == pytest.approx(want, rel=1e-9), (metric, i)
+    ev.close()
+
+
+def test_implicit_score_weights_follow_the_reference():
+    X, trees = _circle(2048)
+    t = _unit_circle_tree(0.9)
+    plans = [host.gpu_fit_plan(t)]
+    rng = np.random.default_rng(3)
+    w = rng.uniform(0.0, 2.0, len(X))
+    w[::7] = 0.0
+    ev = _evaluator(X, w)
+    out, st, _ = ev.implicit_score_batch(plans, 0)
+    ev.close()
+    assert st[0] == 0
+    assert out[0] == pytest.approx(oracle.implicit_score(t, X, w, 0), rel=1e-9)
+    w[5] = -1.0  # a negative weight makes the whole score undefined
+    ev = _evaluator(X, w)
+    out, st, _ = ev.implicit_score_batch(plans, 0)
+    ev.close()
+    assert math.isinf(out[0]) and st[0] == 2
+
+
+def test_implicit_fit_matches_the_oracle_on_the_c5_shape():
+    X, trees = _circle(2048)
+    trees = trees[:16]
+    plans = [host.gpu_fit_plan(t) for t in trees]
+    ev = _evaluator(X)
+    outs, ms = ev.implicit_fit_batch(plans, 8)
+    ev.close()
+    assert ms >= 0
+    want_l, want_c = oracle.implicit_fit_batch(trees, X, None, 8, threads=8, kmax=8)
+    for i, o in enumerate(outs):
+        _say(f"cand {i}: device {o.sse:.10e} status {o.status} oracle {want_l[i]:.10e}")
+        if not math.isfinite(want_l[i]):
+            assert not math.isfinite(o.sse) and o.status == 7
+            continue
+        assert o.status == 0
+        assert o.sse == pytest.approx(want_l[i], rel=1e-6), i
+        # The loss is the geometric RSS of the returned constants.
+        k = plans[i].nconst
+        t = trees[i].with_constants(o.c[:k])
+        d = np.asarray([oracle.signed_distance(t, row) for row in X])
+        assert o.sse == pytest.approx(float(np.sum(d * d)), rel=1e-9)
+
+
+def test_implicit_fit_recovers_the_unit_circle():
+    rng = np.random.default_rng(5)
+    th = rng.uniform(0, 2 * np.pi, 1000)
+    X = np.stack([np.cos(th), np.sin(th)], 1)
+    f = _unit_circle_tree(0.6)
+    ev = _evaluator(X)
+    outs, _ = ev.implicit_fit_batch([host.gpu_fit_plan(f)], 20)
+    ev.close()
+    assert outs[0].status == 0
+    assert outs[0].c[0] == pytest.approx(1.0, abs=1e-6)
+    assert outs[0].sse < 1e-10
+
+
+def test_implicit_iters_zero_and_bad_start():
+    X, trees = _circle(1024)
+    plans = [host.gpu_fit_plan(trees[0]),
+             host.gpu_fit_plan(node(SUB, node(LOG, var(0)), const(0.1)))]
+    ev = _evaluator(X)
+    outs, _ = ev.implicit_fit_batch(plans, 0)
+    ev.close()
+    s0 = oracle.implicit_fit(trees[0], X, None, 0)[0]
+    assert outs[0].sse == pytest.approx(s0, rel=1e-9)
+    assert np.allclose(outs[0].c, plans[0].c0)
+    assert outs[1].status == 7 and math.isinf(outs[1].sse)
+
+
+def _fd_inputs(tree, X, h=1e-6):
+    g = np.zeros_like(X)
+    for j in range(X.shape[1]):
+        Xp, Xm = X.copy(), X.copy()
+        Xp[:, j] += h
+        Xm[:, j] -= h
+        g[:, j] = (oracle.predict(tree, Xp) - oracle.predict(tree, Xm)) / (2 * h)
+    return g
+
+
+def _fd_consts(tree, X, h=1e-6):
+    c = np.asarray(tree.constants())
+    g = np.zeros((X.shape[0], len(c)))
+    for j in range(len(c)):
+        cp, cm = c.copy(), c.copy()
+        cp[j] += h
+        cm[j] -= h
+        g[:, j] = (oracle.predict(tree.with_constants(cp), X) -
+                   oracle.predict(tree.with_constants(cm), X)) / (2 * h)
+    return g
+
+
+def test_adjoint_gradients_match_central_differences():
+    X, trees = _circle(256)
+    trees = trees[:8] + [node(MUL, const(1.3), node(SIN, node(MUL, const(0.7), var(0)))),
+                         node(ADD, node(EXP, node(MUL, const(0.4), var(1))),
+                              node(MUL, const(-0.5), node(COS, var(2))))]
+    ev = _evaluator(X)
+    for i, t in enumerate(trees):
+        plan = host.gpu_fit_plan(t)
+        res = ev.fit_jacobian([plan], plan.c0[None, :], 0, len(X))
+        assert res is not None
+        pred, jac, gx = res
+        want = oracle.predict(t, X)
+        assert np.allclose(pred, want, rtol=1e-5, atol=1e-5), i
+        gxf = _fd_inputs(t, X)
+        gcf = _fd_consts(t, X)
+        ex = np.max(np.abs(gx - gxf) / (1 + np.abs(gxf)))
+        ec = np.max(np.abs(jac[:, :plan.nconst] - gcf) / (1 + np.abs(gcf)))
+        _say(f"tree {i}: input-gradient err {ex:.2e}, constant-gradient err {ec:.2e}")
+        assert ex <= 2e-4 and ec <= 2e-4, (i, ex, ec)
+    ev.close()

@@ -502,7 +502,7 @@ __global__ __launch_bounds__(64) void ft_trial_kernel(
     const FtRows R_ = rowsd[d.rowset];
     const int lane = threadIdx.x;
     const T *cst = cp + (size_t)cand * FT_KMAX;
-    const uint32_t *fc = code + d.fwd_off;
+    const uint32_t *fc = code + d.t_off;
     const T *cols;
     if (sizeof(T) == 4) cols = (const T *)(const void *)R_.cols;
     else cols = (const T *)(const void *)R_.cols64;
@@ -518,7 +518,7 @@ __global__ __launch_bounds__(64) void ft_trial_kernel(
             xrow[r] = live[r] ? row : it.row0;
         }
         T pred[R];
-        ft_value_rows<T, S, R>(fc, d.m, cst, cols, R_.n, xrow, pred);
+        ft_value_rows<T, S, R>(fc, d.t_m, cst, cols, R_.n, xrow, pred);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const double wb = R_.w64 ? R_.w64[xrow[r]] : 1.0;
@@ -603,9 +603,10 @@ __global__ __launch_bounds__(64) void vm_asm_kernel(
         for (int v = 0; v < VM_MAX_VARS; ++v)
             if (v < R.v) {
                 const float *col = R.cols + (size_t)v * (size_t)R.n;
-                VmPack<vm_f2>::set(XV, XV_, v, (vm_f2){col[q0], col[q1]});
+                VmPack<vm_f2>::set(XV, XV_, XV_, XV_, v, (vm_f2){col[q0], col[q1]});
             }
-        const vm_f2 pred = vm_forward<vm_f2, true>(fc, d.m, cst, XV, XV_, U, U_, B, B_);
+        const vm_f2 pred = vm_forward<vm_f2, true>(fc, d.m, cst, XV, XV_, XV_, XV_, U, U_, U_, U_, B,
+                                                   B_, B_, B_);
         double p0 = (double)pred.x, p1 = (double)pred.y;
         if (POLISH) {
             dv16 Ua = (dv16)(0.0), Ub = Ua, Ba = Ua, Bb = Ua;
@@ -614,9 +615,10 @@ __global__ __launch_bounds__(64) void vm_asm_kernel(
             for (int v = 0; v < VM_MAX_VARS; ++v)
                 if (v < R.v) {
                     const double *col = R.cols64 + (size_t)v * (size_t)R.n;
-                    VmPack<vm_d2>::set(Xa, Xb, v, (vm_d2){col[q0], col[q1]});
+                    VmPack<vm_d2>::set(Xa, Xb, Xb, Xb, v, (vm_d2){col[q0], col[q1]});
                 }
-            const vm_d2 pp = vm_forward<vm_d2, false>(fc, d.m, cst64, Xa, Xb, Ua, Ub, Ba, Bb);
+            const vm_d2 pp = vm_forward<vm_d2, false>(fc, d.m, cst64, Xa, Xb, Xb, Xb, Ua, Ub, Ub, Ub,
+                                                      Ba, Bb, Bb, Bb);
             p0 = pp.x;
             p1 = pp.y;
         }
@@ -708,8 +710,9 @@ __global__ __launch_bounds__(64) void vm_asm_kernel(
     }
 }
 
-/* Trial / exact loss on the fit VM: forward only, 4 f32 rows or 2 f64 rows
-   per lane. */
+/* Trial / exact loss on the fit VM: the value-pass stream (inputs in U
+   slots 0..v-1, no tape), 8 f32 rows or 4 f64 rows per lane, with an 8-slot
+   U file of four 64-bit-element vectors. */
 template <typename V>
 __global__ __launch_bounds__(64) void vm_trial_kernel(
     const FtItem *__restrict__ items, const int *__restrict__ list, int nlist,
@@ -718,6 +721,8 @@ __global__ __launch_bounds__(64) void vm_trial_kernel(
     const typename VmScalar<V>::T *__restrict__ cp, double *__restrict__ ploss)
 {
     typedef typename VmScalar<V>::T T;
+    typedef VmPack<V> P;
+    typedef typename P::F8 F;
     constexpr int RR = VmScalar<V>::R;
     const int li = blockIdx.x;
     if (li >= nlist) return;
@@ -729,13 +734,11 @@ __global__ __launch_bounds__(64) void vm_trial_kernel(
     const FtRows R_ = rowsd[d.rowset];
     const int lane = threadIdx.x;
     const T *cst = cp + (size_t)cand * FT_KMAX;
-    const uint2 *fc = (const uint2 *)(code + d.fwd_off);
+    const uint2 *fc = (const uint2 *)(code + d.t_off);
     const T *cols;
     if (sizeof(T) == 4) cols = (const T *)(const void *)R_.cols;
     else cols = (const T *)(const void *)R_.cols64;
-    typedef VmPack<V> P;
-    typename P::F16 U0 = (typename P::F16)(0), U1 = U0, B0 = U0, B1 = U0;
-    typename P::F8 X0 = (typename P::F8)(0), X1 = X0;
+    F U0 = (F)(0), U1 = U0, U2 = U0, U3 = U0, Bd = U0;
     double lsum = 0.0;
     const int row_end = it.row0 + it.rows;
     for (int base = it.row0; base < row_end; base += 64 * RR) {
@@ -754,9 +757,10 @@ __global__ __launch_bounds__(64) void vm_trial_kernel(
                 V xv;
 #pragma unroll
                 for (int r = 0; r < RR; ++r) xv[r] = col[xrow[r]];
-                P::set(X0, X1, v, xv);
+                P::set(U0, U1, U2, U3, v, xv);
             }
-        const V pred = vm_forward<V, false>(fc, d.m, cst, X0, X1, U0, U1, B0, B1);
+        const V pred = vm_forward<V, false>(fc, d.t_m, cst, U0, U1, U2, U3, U0, U1, U2, U3, Bd, Bd,
+                                            Bd, Bd);
 #pragma unroll
         for (int r = 0; r < RR; ++r) {
             const double wb = R_.w64 ? R_.w64[xrow[r]] : 1.0;
@@ -1264,8 +1268,8 @@ void rc_fit_state_free(RcCtx *c)
 
 int rc_rows_ensure_f64(RcCtx *c, RcRows *r);
 extern "C" int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int col_bytes,
-                                 unsigned int *fwd, int cap_f, unsigned int *rev, int cap_r,
-                                 int *info);
+                                 unsigned int flags, unsigned int *fwd, int cap_f,
+                                 unsigned int *rev, int cap_r, int *info);
 int rc_fit64_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const int *pcount, int nprog,
                  int kmax, const double *c0, const std::vector<int> &rowset,
                  const std::vector<int> &metric, const std::vector<int> &pre, int iters,
@@ -1390,9 +1394,10 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         if (pre[i] >= 0) continue;
         const int words = (int)(off[i + 1] - off[i]);
         int info[6] = {0, 0, 0, 0, 0, 0};
-        const int vmrc = rch_fitvm_compile(prog + off[i], words, pcount[i], c->sets[d.rowset].v,
-                                           XS2 * 4, vf.data(), (int)vf.size(), vr.data(),
-                                           (int)vr.size(), info);
+        const int nv = c->sets[d.rowset].v;
+        const int vmrc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, XS2 * 4, 0u,
+                                           vf.data(), (int)vf.size(), vr.data(), (int)vr.size(),
+                                           info);
         if (vmrc == 0) {
             if (code.size() & 1) code.push_back(0);
             d.vm = 1;
@@ -1410,13 +1415,30 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
             d.m = (int)progs[i].fwd.size();
             d.nrev = (int)progs[i].rev.size();
         }
+        /* The value-pass stream: the fit VM's value-only form (inputs in U,
+           8-slot file) when it fits, else the general interpreter's. */
+        int tinfo[6] = {0, 0, 0, 0, 0, 0};
+        const int vorc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, XS2 * 4,
+                                           VMC_INPUTS_IN_U | VMC_VALUE_ONLY | VMC_SMALL_U,
+                                           vf.data(), (int)vf.size(), nullptr, 0, tinfo);
+        int tcls;
+        if (vorc == 0) {
+            if (code.size() & 1) code.push_back(0);
+            d.t_off = (uint32_t)code.size();
+            code.insert(code.end(), vf.begin(), vf.begin() + 2 * tinfo[0]);
+            d.t_m = tinfo[0];
+            tcls = 0;
+        } else {
+            d.t_off = (uint32_t)code.size();
+            code.insert(code.end(), progs[i].fwd.begin(), progs[i].fwd.end());
+            d.t_m = (int)progs[i].fwd.size();
+            tcls = progs[i].max_stack > SMALL_S ? 2 : 1;
+        }
         const int n = c->sets[d.rowset].n;
         d.item0 = (int)items.size();
         const bool big = prog_is_big(progs[i]);
-        const bool tbig = progs[i].max_stack > SMALL_S;
         const int kc = d.k + 2 <= 16 ? 0 : 1;
         const int cls = d.vm ? kc : (big ? 4 : 2) + kc;
-        const int tcls = d.vm ? 0 : (tbig ? 2 : 1);
         maxK = std::max(maxK, d.k + 2);
         for (int r0 = 0; r0 < n; r0 += TILE_ROWS) {
             FtItem it;
@@ -1575,14 +1597,14 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
             const int *L = d_lists + tofs[q];
             if (!exact) {
                 if (q == 0)
-                    vm_trial_kernel<vm_f4><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
+                    vm_trial_kernel<vm_f8><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
                 else if (q == 1)
                     ft_trial_kernel<float, false><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
                 else
                     ft_trial_kernel<float, true><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
             } else {
                 if (q == 0)
-                    vm_trial_kernel<vm_d2><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
+                    vm_trial_kernel<vm_d4><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
                 else if (q == 1)
                     ft_trial_kernel<double, false><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
                 else
@@ -1849,18 +1871,6 @@ int rc_fit_jacobian(void *ctx, const unsigned int *prog, const unsigned int *off
     if (dj) (void)hipFree(dj);
     if (dg) (void)hipFree(dg);
     return rc;
-}
-
-float rc_implicit_score_batch(void *, const unsigned int *, const unsigned int *, const int *, int,
-                              int, const double *, int, double *, int *)
-{
-    return -1.0f;
-}
-
-float rc_implicit_fit_batch(void *, const unsigned int *, const unsigned int *, const int *, int, int,
-                            const double *, int, double *, double *, int *)
-{
-    return -1.0f;
 }
 
 } /* extern "C" */

@@ -53,7 +53,8 @@ struct FtCand {
     int item0;   /* first work item */
     int nitems;
     int vm;      /* 1: fit VM code (word pairs, m / nrev count pairs) */
-    int pad[2];
+    uint32_t t_off; /* value-pass stream (trial and exact losses) */
+    int t_m;        /* its instruction count */
 };
 
 /* One resident row set as the kernels see it. */

@@ -76,6 +76,8 @@ struct Unit {
 struct Compiler {
     std::vector<Nd> nd;
     int nvars = 0;
+    bool vo = false;       /* value only: no tape */
+    int usink = VM_U_SINK; /* the U sink slot */
     int depth = 0;
     std::vector<int> spill;
     bool ufree[VM_U_SLOTS];
@@ -85,17 +87,20 @@ struct Compiler {
 
     int ualloc()
     {
-        for (int s = 0; s < VM_U_SINK; ++s)
+        for (int s = 0; s < usink; ++s)
             if (ufree[s]) {
                 ufree[s] = false;
                 umax = std::max(umax, s + 1);
                 return s;
             }
         overflow = true;
-        return VM_U_SINK;
+        return usink;
     }
+    /* A tape slot: none in value-only code. */
+    int talloc() { return vo ? usink : ualloc(); }
     int balloc()
     {
+        if (vo) return VM_B_SINK;
         if (bnext >= VM_B_SINK) {
             overflow = true;
             return VM_B_SINK;
@@ -134,7 +139,7 @@ struct Compiler {
 
     void leaf_push(int i)
     {
-        const int us = depth > 0 ? ualloc() : VM_U_SINK;
+        const int us = depth > 0 ? ualloc() : usink;
         if (depth > 0) spill.push_back(us);
         Unit u = blank();
         if (nd[i].op == O_CONST) {
@@ -160,8 +165,8 @@ struct Compiler {
         if (!is_bin(N.op)) {
             const int ch = N.l;
             emit(ch);
-            const bool need = nd[ch].on;
-            const int us = need ? ualloc() : VM_U_SINK;
+            const bool need = nd[ch].on && !vo;
+            const int us = need ? talloc() : usink;
             Unit u = blank();
             u.fx = VMF_UN | (need ? VMF_NEED : 0u) | un_bits(N.op);
             u.fy = yf(-1, us, 0, VM_B_SINK, -1, N.op);
@@ -187,7 +192,7 @@ struct Compiler {
         const Nd &NR = nd[R];
         /* x * c */
         if (N.op == O_MUL && NR.op == O_CONST && NL.op == O_VAR) {
-            const int us = depth > 0 ? ualloc() : VM_U_SINK;
+            const int us = depth > 0 ? ualloc() : usink;
             if (depth > 0) spill.push_back(us);
             ++depth;
             Unit u = blank();
@@ -205,8 +210,8 @@ struct Compiler {
             nd[NL.l].op != O_CONST) {
             const int g = NL.l;
             emit(g);
-            const bool need = nd[g].on;
-            const int us = ualloc();
+            const bool need = nd[g].on && !vo;
+            const int us = talloc();
             const int bs = need ? balloc() : VM_B_SINK;
             Unit u = blank();
             u.fx = VMF_UNMC | (need ? VMF_NEED : 0u) | un_bits(NL.op);
@@ -223,7 +228,7 @@ struct Compiler {
         /* (.) * c */
         if (N.op == O_MUL && NR.op == O_CONST && !leaf(L)) {
             emit(L);
-            const int us = ualloc();
+            const int us = talloc();
             Unit u = blank();
             u.fx = VMF_MULC;
             u.fy = yf((int)NR.slot, us, 0, VM_B_SINK, -1, 0);
@@ -240,7 +245,7 @@ struct Compiler {
             emit(L);
             Unit u = blank();
             u.fx = VMF_ADDC;
-            u.fy = yf((int)NR.slot, VM_U_SINK, 0, VM_B_SINK, -1, 0);
+            u.fy = yf((int)NR.slot, usink, 0, VM_B_SINK, -1, 0);
             u.rkind = NL.on ? VMR_ADDC_N : VMR_ADDC_P;
             u.gl = (int)NR.slot;
             u.nodes_next = NL.on ? 1 : 0;
@@ -252,7 +257,7 @@ struct Compiler {
             emit(L);
             Unit u = blank();
             u.fx = VMF_MULV;
-            u.fy = yf(-1, VM_U_SINK, 0, VM_B_SINK, (int)NR.slot, 0);
+            u.fy = yf(-1, usink, 0, VM_B_SINK, (int)NR.slot, 0);
             if (NL.on) {
                 u.rkind = VMR_MULV;
                 u.xv = (int)NR.slot;
@@ -263,7 +268,7 @@ struct Compiler {
         }
         /* General binary node. */
         const Kind lk = kind(L), rk = kind(R);
-        const bool tape = N.op == O_MUL || N.op == O_DIV || N.op == O_POW;
+        const bool tape = !vo && (N.op == O_MUL || N.op == O_DIV || N.op == O_POW);
         const bool dl = tape && lk != K_OFF, dr = tape && rk != K_OFF;
         Unit u = blank();
         int ur = 0;
@@ -283,7 +288,7 @@ struct Compiler {
             spill.pop_back();
             --depth;
         }
-        const int us = dl ? ualloc() : VM_U_SINK;
+        const int us = dl ? ualloc() : usink;
         const int bs = dr ? balloc() : VM_B_SINK;
         if (!(fx & (VMF_RC | VMF_RV))) ufree[ur] = true;
         if (dl) fx |= VMF_D0;
@@ -321,18 +326,22 @@ struct Compiler {
 extern "C" {
 
 /* Compile one candidate for the fit VM.  rec: words/4 post-order records;
-   p constants; nvars inputs; col_bytes: LDS bytes between gradient columns.
+   p constants; nvars inputs; col_bytes: LDS bytes between gradient columns;
+   flags: VMC_* options.
    fwd / rev receive word pairs (2 words each).  info[0..5] = forward pairs,
    reverse pairs, U slots, B slots, saved-adjoint slots, peak value depth.
    Returns 0, or -1 malformed records, -2 unsupported operator, -3 the
    program does not fit the VM, -4 an output buffer is too small. */
 int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int col_bytes,
-                      unsigned int *fwd, int cap_f, unsigned int *rev, int cap_r, int *info)
+                      unsigned int flags, unsigned int *fwd, int cap_f, unsigned int *rev,
+                      int cap_r, int *info)
 {
     if (!rec || words <= 0 || words % 4 != 0 || p < 0 || nvars < 0 || col_bytes <= 0) return -1;
     const int m = words / 4;
     Compiler C;
     C.nvars = nvars;
+    C.vo = (flags & VMC_VALUE_ONLY) != 0;
+    C.usink = (flags & VMC_SMALL_U) ? VM_U_SINK_SMALL : VM_U_SINK;
     C.nd.resize((size_t)m);
     std::vector<int> st;
     bool unsupported = false;
@@ -381,12 +390,16 @@ int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int 
             std::swap(N.l, N.r);
     }
     for (int s = 0; s < VM_U_SLOTS; ++s) C.ufree[s] = true;
-    int peak = 0;
-    (void)peak;
+    if (flags & VMC_INPUTS_IN_U) {
+        /* the inputs occupy U slots 0..nvars-1 for the whole pass */
+        if (nvars >= C.usink) return -3;
+        for (int s = 0; s < nvars; ++s) C.ufree[s] = false;
+        C.umax = nvars;
+    }
     C.emit(m - 1);
     if (C.overflow) return -3;
     /* A lone constant: its gradient is the seed. */
-    if (m == 1) {
+    if (m == 1 && !C.vo) {
         Unit &u = C.units.back();
         u.rkind = VMR_ADDC_P;
         u.gl = (int)C.nd[0].slot;
@@ -396,7 +409,7 @@ int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int 
        slots. */
     std::vector<uint32_t> rv;
     int adepth = 0, apeak = 0;
-    for (int i = (int)C.units.size() - 1; i >= 0; --i) {
+    for (int i = C.vo ? -1 : (int)C.units.size() - 1; i >= 0; --i) {
         const Unit &u = C.units[(size_t)i];
         if (!u.rkind) continue;
         uint32_t x = u.rkind, y = 0;
@@ -410,6 +423,8 @@ int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int 
             ar = adepth > 0 ? --adepth : VM_A_SINK;
             if (u.rkind == VMR_GEN) x |= VMR_POP;
         }
+        if (u.gl >= 0 && (u.gl + 1) * col_bytes >= (1 << (32 - VMR_GOFS_SHIFT))) return -3;
+        if (u.gr >= 0 && (u.gr + 1) * col_bytes >= (1 << (32 - VMR_GOFS_SHIFT))) return -3;
         if (u.gl >= 0) x |= (uint32_t)(u.gl * col_bytes) << VMR_GOFS_SHIFT;
         if (u.cslot >= 0) y |= (uint32_t)(u.cslot * 4) & 0xffu;
         y |= ((uint32_t)(u.ut < 0 ? VM_U_SINK : u.ut) & 0xfu) << 8;
@@ -427,7 +442,7 @@ int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int 
     }
     const int nf = (int)C.units.size();
     const int nr = (int)rv.size() / 2;
-    if (!fwd || !rev || 2 * nf > cap_f || 2 * nr > cap_r) return -4;
+    if (!fwd || (!rev && nr > 0) || 2 * nf > cap_f || 2 * nr > cap_r) return -4;
     for (int i = 0; i < nf; ++i) {
         fwd[2 * i] = C.units[(size_t)i].fx;
         fwd[2 * i + 1] = C.units[(size_t)i].fy;

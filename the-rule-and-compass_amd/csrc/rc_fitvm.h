@@ -37,6 +37,12 @@
 #define VM_A_SINK 7
 #define VM_MAX_VARS 8
 
+/* Compiler options (rch_fitvm_compile flags). */
+#define VMC_INPUTS_IN_U 1u /* inputs live in U slots 0..nvars-1 (no X file)  */
+#define VMC_VALUE_ONLY 2u  /* forward only: no tape, no reverse stream       */
+#define VMC_SMALL_U 4u     /* an 8-slot U file: slot 7 is the sink           */
+#define VM_U_SINK_SMALL 7
+
 /* ---- forward handlers (x bits 0..9) ------------------------------------ */
 /* F_UNMC  f = un(tos); U[us] = f; [NEED] B[bs] = f'(tos, f); tos = f * c   */
 #define VMF_UNMC (1u << 0)

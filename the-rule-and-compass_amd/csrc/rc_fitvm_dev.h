@@ -40,25 +40,42 @@ __device__ __forceinline__ vm_u64 vm_pack(vm_f2 x)
     return (vm_u64)__float_as_uint(x.x) | ((vm_u64)__float_as_uint(x.y) << 32);
 }
 
-/* Register files are plain local vectors of 64-bit elements (a struct
-   holding them would be demoted to scratch): every file is a pair (f0, f1)
-   of such vectors.  A vm_f2 pack keeps rows 0-1 in f0 (f1 unused), a vm_f4
-   pack rows 0-1 in f0 and 2-3 in f1, a vm_d2 pack row 0 in f0 and row 1 in
-   f1.  Both halves of a slot share one index, so the compiler moves them
-   under one s_set_gpr_idx bracket. */
+/* Register files are plain local vectors of 64-bit elements (a struct or an
+   array holding them would be demoted to scratch): a file is up to four such
+   vectors (f0..f3) indexed by one slot number.  A vm_f2 pack keeps rows 0-1
+   in f0, vm_f4 rows 0-1 / 2-3 in f0 / f1, vm_f8 four row pairs in f0..f3;
+   a vm_d2 pack keeps one f64 row in each of f0, f1, vm_d4 one in each of
+   f0..f3.  All halves of a slot share one index, so the compiler moves them
+   under one s_set_gpr_idx bracket.  The slot index is masked to the file
+   size, so an 8-slot file maps the 16-slot sink onto its own sink, slot 7. */
+typedef float vm_f8 __attribute__((ext_vector_type(8)));
+typedef double vm_d4 __attribute__((ext_vector_type(4)));
+
+template <typename F> struct VmSlots;
+template <> struct VmSlots<vm_u64x8> { enum { N = 8 }; };
+template <> struct VmSlots<vm_u64x16> { enum { N = 16 }; };
+template <> struct VmSlots<dv8> { enum { N = 8 }; };
+template <> struct VmSlots<dv16> { enum { N = 16 }; };
+template <typename F> __device__ __forceinline__ unsigned vm_slot(unsigned i)
+{
+    return i & (unsigned)(VmSlots<F>::N - 1);
+}
+
 template <typename V> struct VmPack;
 template <> struct VmPack<vm_f2> {
     typedef float T;
     typedef vm_u64x16 F16;
     typedef vm_u64x8 F8;
     enum { R = 2 };
-    template <typename F> static __device__ __forceinline__ vm_f2 get(const F &f0, const F &, unsigned i)
+    template <typename F>
+    static __device__ __forceinline__ vm_f2 get(const F &f0, const F &, const F &, const F &, unsigned i)
     {
-        return vm_unpack(f0[i]);
+        return vm_unpack(f0[vm_slot<F>(i)]);
     }
-    template <typename F> static __device__ __forceinline__ void set(F &f0, F &, unsigned i, vm_f2 x)
+    template <typename F>
+    static __device__ __forceinline__ void set(F &f0, F &, F &, F &, unsigned i, vm_f2 x)
     {
-        f0[i] = vm_pack(x);
+        f0[vm_slot<F>(i)] = vm_pack(x);
     }
 };
 template <> struct VmPack<vm_f4> {
@@ -66,15 +83,43 @@ template <> struct VmPack<vm_f4> {
     typedef vm_u64x16 F16;
     typedef vm_u64x8 F8;
     enum { R = 4 };
-    template <typename F> static __device__ __forceinline__ vm_f4 get(const F &f0, const F &f1, unsigned i)
+    template <typename F>
+    static __device__ __forceinline__ vm_f4 get(const F &f0, const F &f1, const F &, const F &, unsigned i)
     {
-        const vm_f2 a = vm_unpack(f0[i]), b = vm_unpack(f1[i]);
+        const unsigned j = vm_slot<F>(i);
+        const vm_f2 a = vm_unpack(f0[j]), b = vm_unpack(f1[j]);
         return (vm_f4){a.x, a.y, b.x, b.y};
     }
-    template <typename F> static __device__ __forceinline__ void set(F &f0, F &f1, unsigned i, vm_f4 x)
+    template <typename F>
+    static __device__ __forceinline__ void set(F &f0, F &f1, F &, F &, unsigned i, vm_f4 x)
     {
-        f0[i] = vm_pack((vm_f2){x.x, x.y});
-        f1[i] = vm_pack((vm_f2){x.z, x.w});
+        const unsigned j = vm_slot<F>(i);
+        f0[j] = vm_pack((vm_f2){x[0], x[1]});
+        f1[j] = vm_pack((vm_f2){x[2], x[3]});
+    }
+};
+template <> struct VmPack<vm_f8> {
+    typedef float T;
+    typedef vm_u64x16 F16;
+    typedef vm_u64x8 F8;
+    enum { R = 8 };
+    template <typename F>
+    static __device__ __forceinline__ vm_f8 get(const F &f0, const F &f1, const F &f2, const F &f3,
+                                                unsigned i)
+    {
+        const unsigned j = vm_slot<F>(i);
+        const vm_f2 a = vm_unpack(f0[j]), b = vm_unpack(f1[j]), c = vm_unpack(f2[j]),
+                    d = vm_unpack(f3[j]);
+        return (vm_f8){a.x, a.y, b.x, b.y, c.x, c.y, d.x, d.y};
+    }
+    template <typename F>
+    static __device__ __forceinline__ void set(F &f0, F &f1, F &f2, F &f3, unsigned i, vm_f8 x)
+    {
+        const unsigned j = vm_slot<F>(i);
+        f0[j] = vm_pack((vm_f2){x[0], x[1]});
+        f1[j] = vm_pack((vm_f2){x[2], x[3]});
+        f2[j] = vm_pack((vm_f2){x[4], x[5]});
+        f3[j] = vm_pack((vm_f2){x[6], x[7]});
     }
 };
 template <> struct VmPack<vm_d2> {
@@ -82,14 +127,40 @@ template <> struct VmPack<vm_d2> {
     typedef dv16 F16;
     typedef dv8 F8;
     enum { R = 2 };
-    template <typename F> static __device__ __forceinline__ vm_d2 get(const F &f0, const F &f1, unsigned i)
+    template <typename F>
+    static __device__ __forceinline__ vm_d2 get(const F &f0, const F &f1, const F &, const F &, unsigned i)
     {
-        return (vm_d2){f0[i], f1[i]};
+        const unsigned j = vm_slot<F>(i);
+        return (vm_d2){f0[j], f1[j]};
     }
-    template <typename F> static __device__ __forceinline__ void set(F &f0, F &f1, unsigned i, vm_d2 x)
+    template <typename F>
+    static __device__ __forceinline__ void set(F &f0, F &f1, F &, F &, unsigned i, vm_d2 x)
     {
-        f0[i] = x.x;
-        f1[i] = x.y;
+        const unsigned j = vm_slot<F>(i);
+        f0[j] = x[0];
+        f1[j] = x[1];
+    }
+};
+template <> struct VmPack<vm_d4> {
+    typedef double T;
+    typedef dv16 F16;
+    typedef dv8 F8;
+    enum { R = 4 };
+    template <typename F>
+    static __device__ __forceinline__ vm_d4 get(const F &f0, const F &f1, const F &f2, const F &f3,
+                                                unsigned i)
+    {
+        const unsigned j = vm_slot<F>(i);
+        return (vm_d4){f0[j], f1[j], f2[j], f3[j]};
+    }
+    template <typename F>
+    static __device__ __forceinline__ void set(F &f0, F &f1, F &f2, F &f3, unsigned i, vm_d4 x)
+    {
+        const unsigned j = vm_slot<F>(i);
+        f0[j] = x[0];
+        f1[j] = x[1];
+        f2[j] = x[2];
+        f3[j] = x[3];
     }
 };
 
@@ -203,40 +274,40 @@ template <typename V> __device__ __forceinline__ V vm_partial(uint32_t x, uint32
    reverse stream reads are left in U and B.  Every step writes U once (the
    sink slot when it has nothing to keep), so no branch join merges two
    versions of the file. */
-template <typename V, bool TAPE>
+template <typename V, bool TAPE, typename FX, typename FU, typename FB>
 __device__ __forceinline__ V vm_forward(const uint2 *__restrict__ fc, int nf,
                                         const typename VmScalar<V>::T *__restrict__ cst,
-                                        const typename VmPack<V>::F8 &X0,
-                                        const typename VmPack<V>::F8 &X1,
-                                        typename VmPack<V>::F16 &U0, typename VmPack<V>::F16 &U1,
-                                        typename VmPack<V>::F16 &B0, typename VmPack<V>::F16 &B1)
+                                        const FX &X0, const FX &X1, const FX &X2, const FX &X3,
+                                        FU &U0, FU &U1, FU &U2, FU &U3, FB &B0, FB &B1, FB &B2,
+                                        FB &B3)
 {
     typedef VmPack<V> P;
     V tos = vm_bcast<V>(0);
-    for (int i = 0; i < nf; ++i) {
-        const uint2 w = fc[i];
+    const uint32_t end = 8u * (uint32_t)nf;
+    for (uint32_t o = 0; o < end; o += 8u) {
+        const uint2 w = *(const uint2 *)((const char *)fc + o);
         const uint32_t x = w.x, y = w.y;
         V nt, uv = tos;
         uint32_t us = VMF_Y_US(y);
         if (x & VMF_UNMC) {
             const V f = vm_unary<V>(x, y, tos);
-            if (TAPE && (x & VMF_NEED)) P::set(B0, B1, VMF_Y_BS(y), vm_partial<V>(x, y, tos, f));
+            if (TAPE && (x & VMF_NEED)) P::set(B0, B1, B2, B3, VMF_Y_BS(y), vm_partial<V>(x, y, tos, f));
             uv = f;
             nt = f * vm_const(cst, y);
         } else if (x & VMF_VMC) {
-            nt = P::get(X0, X1, VMF_Y_XV(y)) * vm_const(cst, y);
+            nt = P::get(X0, X1, X2, X3, VMF_Y_XV(y)) * vm_const(cst, y);
         } else if (x & VMF_ADD) {
-            nt = P::get(U0, U1, VMF_Y_UR(y)) + tos;
+            nt = P::get(U0, U1, U2, U3, VMF_Y_UR(y)) + tos;
         } else if (x & VMF_MULC) {
             nt = tos * vm_const(cst, y);
         } else if (x & VMF_PUSHV) {
-            nt = P::get(X0, X1, VMF_Y_XV(y));
+            nt = P::get(X0, X1, X2, X3, VMF_Y_XV(y));
         } else if (x & VMF_UN) {
             const V f = vm_unary<V>(x, y, tos);
             if (TAPE && (x & VMF_NEED)) uv = vm_partial<V>(x, y, tos, f);
             nt = f;
         } else if (x & VMF_MULV) {
-            nt = tos * P::get(X0, X1, VMF_Y_XV(y));
+            nt = tos * P::get(X0, X1, X2, X3, VMF_Y_XV(y));
         } else if (x & VMF_ADDC) {
             nt = tos + vm_const(cst, y);
         } else if (x & VMF_PUSHC) {
@@ -248,9 +319,9 @@ __device__ __forceinline__ V vm_forward(const uint2 *__restrict__ fc, int nf,
                 r = vm_bcast<V>(vm_const(cst, y));
             } else if (x & VMF_RV) {
                 l = tos;
-                r = P::get(X0, X1, VMF_Y_XV(y));
+                r = P::get(X0, X1, X2, X3, VMF_Y_XV(y));
             } else {
-                l = P::get(U0, U1, VMF_Y_UR(y));
+                l = P::get(U0, U1, U2, U3, VMF_Y_UR(y));
                 r = tos;
             }
             const unsigned op = VMF_Y_OP(y);
@@ -262,10 +333,10 @@ __device__ __forceinline__ V vm_forward(const uint2 *__restrict__ fc, int nf,
                 V d0, d1;
                 vm_binary_partials<V>(op, l, r, d0, d1);
                 uv = d0;
-                if (x & VMF_D1) P::set(B0, B1, VMF_Y_BS(y), d1);
+                if (x & VMF_D1) P::set(B0, B1, B2, B3, VMF_Y_BS(y), d1);
             }
         }
-        P::set(U0, U1, us, uv);
+        P::set(U0, U1, U2, U3, us, uv);
         tos = nt;
     }
     return tos;
@@ -311,43 +382,44 @@ __device__ __forceinline__ vm_f2 vm_reverse(const uint2 *__restrict__ rc, int nr
     typedef VmPack<vm_f2> P;
     vm_u64x8 Ad = (vm_u64x8)(0), none;
     vm_f2 a = seed, fin = {0.0f, 0.0f};
-    for (int i = 0; i < nr; ++i) {
-        const uint2 w = rc[i];
+    const uint32_t end = 8u * (uint32_t)nr;
+    for (uint32_t o = 0; o < end; o += 8u) {
+        const uint2 w = *(const uint2 *)((const char *)rc + o);
         const uint32_t x = w.x, y = w.y;
         if (x & VMR_UNMC_N) {
-            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, VMR_Y_UT(y))), fin);
-            a = vm_zmul(vm_zmul(a, vm_const(cst, y)), P::get(B, B, VMR_Y_BT(y)));
+            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, U, U, VMR_Y_UT(y))), fin);
+            a = vm_zmul(vm_zmul(a, vm_const(cst, y)), P::get(B, B, B, B, VMR_Y_BT(y)));
         } else if (x & VMR_VMC) {
-            vm_gwrite(xl, x, vm_zmul(a, P::get(X, X, VMR_Y_XV(y))), fin);
-            a = P::get(Ad, Ad, VMR_Y_AR(y));
+            vm_gwrite(xl, x, vm_zmul(a, P::get(X, X, X, X, VMR_Y_XV(y))), fin);
+            a = P::get(Ad, Ad, Ad, Ad, VMR_Y_AR(y));
         } else if (x & VMR_PUSH) {
-            P::set(Ad, none, VMR_Y_AS(y), a);
+            P::set(Ad, none, none, none, VMR_Y_AS(y), a);
         } else if (x & VMR_MULC_N) {
-            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, VMR_Y_UT(y))), fin);
+            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, U, U, VMR_Y_UT(y))), fin);
             a = vm_zmul(a, vm_const(cst, y));
         } else if (x & VMR_UNMC_P) {
-            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, VMR_Y_UT(y))), fin);
-            a = P::get(Ad, Ad, VMR_Y_AR(y));
+            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, U, U, VMR_Y_UT(y))), fin);
+            a = P::get(Ad, Ad, Ad, Ad, VMR_Y_AR(y));
         } else if (x & VMR_UN) {
-            a = vm_zmul(a, P::get(U, U, VMR_Y_UT(y)));
+            a = vm_zmul(a, P::get(U, U, U, U, VMR_Y_UT(y)));
         } else if (x & VMR_MULC_P) {
-            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, VMR_Y_UT(y))), fin);
-            a = P::get(Ad, Ad, VMR_Y_AR(y));
+            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, U, U, VMR_Y_UT(y))), fin);
+            a = P::get(Ad, Ad, Ad, Ad, VMR_Y_AR(y));
         } else if (x & VMR_MULV) {
-            a = vm_zmul(a, P::get(X, X, VMR_Y_XV(y)));
+            a = vm_zmul(a, P::get(X, X, X, X, VMR_Y_XV(y)));
         } else if (x & VMR_ADDC_N) {
             vm_gwrite(xl, x, a, fin);
         } else if (x & VMR_ADDC_P) {
             vm_gwrite(xl, x, a, fin);
-            a = P::get(Ad, Ad, VMR_Y_AR(y));
+            a = P::get(Ad, Ad, Ad, Ad, VMR_Y_AR(y));
         } else {
             /* R_GEN: two word pairs. */
-            const uint32_t x2 = rc[i + 1].x;
-            ++i;
+            o += 8u;
+            const uint32_t x2 = *(const uint32_t *)((const char *)rc + o);
             const unsigned op = VMR_Y_OP(y);
             vm_f2 gl, gr;
             if (x & VMR_UNARY) {
-                gl = vm_zmul(a, P::get(U, U, VMR_Y_UT(y)));
+                gl = vm_zmul(a, P::get(U, U, U, U, VMR_Y_UT(y)));
                 gr = gl;
             } else if (op == OP_ADD) {
                 gl = a;
@@ -356,20 +428,20 @@ __device__ __forceinline__ vm_f2 vm_reverse(const uint2 *__restrict__ rc, int nr
                 gl = a;
                 gr = -a;
             } else {
-                gl = vm_zmul(a, P::get(U, U, VMR_Y_UT(y)));
-                gr = vm_zmul(a, P::get(B, B, VMR_Y_BT(y)));
+                gl = vm_zmul(a, P::get(U, U, U, U, VMR_Y_UT(y)));
+                gr = vm_zmul(a, P::get(B, B, B, B, VMR_Y_BT(y)));
             }
             if (x & VMR_LCONST) vm_gwrite(xl, x, gl, fin);
             if (x & VMR_RCONST) vm_gwrite(xl, x2, gr, fin);
             if ((x & VMR_LNODE) && (x & VMR_RNODE)) {
-                P::set(Ad, none, VMR_Y_AS(y), gl);
+                P::set(Ad, none, none, none, VMR_Y_AS(y), gl);
                 a = gr;
             } else if (x & VMR_LNODE) {
                 a = gl;
             } else if (x & VMR_RNODE) {
                 a = gr;
             } else {
-                a = P::get(Ad, Ad, VMR_Y_AR(y));
+                a = P::get(Ad, Ad, Ad, Ad, VMR_Y_AR(y));
             }
         }
     }

@@ -1,0 +1,682 @@
+/*
+ * Implicit relations on gfx950: f(x) = 0 scored and fitted by the signed
+ * geometric distance d = f / ||grad_x f|| (implicit.rs:59-411, CPU only in
+ * the reference; here a device arm behind rc_implicit_score_batch and
+ * rc_implicit_fit_batch).
+ *
+ * Semantics per row, all in f64 as the reference:
+ *   f        expr::eval with its domain guards (expr.rs:1054-1330),
+ *   grad_x f the adjoint sweep over the fitter's value_of node values
+ *            (fit.rs:1211-1365, 1453-1671) with the input gradients as
+ *            output (fit.rs:3995 input_gradients),
+ *   d        undefined when f, a gradient component, the norm or d is not
+ *            finite, or the norm is not positive (signed_distance 68-96).
+ *
+ * Layout.  Work items are (candidate, 8192-row tile) with one wave each; a
+ * lane walks its rows of the tile and evaluates the candidate's post-order
+ * records (the rch_fit_plan form) with per-lane value and adjoint arrays.
+ * Three kinds of pass:
+ *   SCORE  sum w * |d| (metric 0) or w * |f| (metric 1) and sum w;
+ *   RSS    sum (sqrt(w) d)^2 at one constant vector;
+ *   JAC    the central-difference Jacobian of r = sqrt(w) d in the constants
+ *          (2k + 1 distances per row, h = 1e-6 (1 + |c_j|), implicit.rs:
+ *          320-345) and the augmented Gram [J | r]^T [J | r]: the rows of a
+ *          64-row step are staged in LDS and every Gram entry is owned by one
+ *          lane that adds the 64 products in row order.
+ * Partials are per item and reduced on the host in item order, so a
+ * candidate's result does not depend on the batch.  The LM control
+ * (Gaussian elimination with partial pivoting, lambda x6 / /8, clamp to
+ * +-1e6, implicit.rs:215-253, 277-409) runs on the host between batched
+ * passes: one JAC and one RSS launch per iteration for the whole batch.
+ */
+#include "rc_device.h"
+#include "rc_internal.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+namespace {
+
+constexpr int IM_TILE = 8192;
+constexpr int IM_MAX_NODES = 96;
+constexpr int IM_KMAX = 20;
+constexpr int IM_STATS = (IM_KMAX + 1) * (IM_KMAX + 2) / 2; /* [J | r] lower triangle */
+constexpr double IM_INV_SQRT_PI = 0.5641895835477563;
+
+enum { IM_SCORE = 0, IM_RSS = 1, IM_JAC = 2 };
+
+struct ImCand {
+    uint32_t off; /* first record word */
+    int m;        /* nodes */
+    int k;        /* constants */
+    int pad;
+};
+
+struct ImItem {
+    int cand;
+    int row0;
+    int rows;
+    int pad;
+};
+
+/* value_of (fit.rs:1211-1365) for the fit operator set. */
+__device__ __forceinline__ double im_value(uint32_t op, double a, double b)
+{
+    switch (op) {
+        case OP_ADD: return a + b;
+        case OP_SUB: return a - b;
+        case OP_MUL: return a * b;
+        case OP_DIV: return fabs(b) < 1e-12 ? RC_F64_NAN : a / b;
+        case OP_POW: return pow(a, b);
+        case OP_NEG: return -a;
+        case OP_EXP: return a > 700.0 ? __builtin_inf() : exp(a);
+        case OP_LOG: return log(a);
+        case OP_ABS: return fabs(a);
+        case OP_SQRT: return sqrt(a);
+        case OP_SIN: return sin(a);
+        case OP_COS: return cos(a);
+        case OP_TAN: return tan(a);
+        case OP_SINH: return sinh(a);
+        case OP_COSH: return cosh(a);
+        case OP_TANH: return tanh(a);
+        case OP_SIG: return 1.0 / (1.0 + exp(-a));
+        case OP_GAUSS: return exp(-a * a);
+        case OP_SQUARE: return a * a;
+        case OP_CUBE: return a * a * a;
+        case OP_ERF: return erf(a);
+        case OP_ERFC: return erfc(a);
+        default: return RC_F64_NAN;
+    }
+}
+
+/* Whether expr::eval's guard fires where value_of does not (LOG of a
+   non-positive, SQRT of a negative, a non-finite TAN, POW of a negative base
+   to a non-integer power): then f comes from the guarded evaluation. */
+__device__ __forceinline__ bool im_guard(uint32_t op, double a, double b, double v)
+{
+    switch (op) {
+        case OP_LOG: return !(a > 0.0);
+        case OP_SQRT: return !(a >= 0.0);
+        case OP_TAN: return !isfinite(v);
+        case OP_POW: return a < 0.0 && fabs(b - round(b)) > 1e-9;
+        default: return false;
+    }
+}
+
+__device__ __forceinline__ double im_guarded(uint32_t op, double a, double b)
+{
+    switch (op) {
+        case OP_LOG: return a > 0.0 ? log(a) : RC_F64_NAN;
+        case OP_SQRT: return a >= 0.0 ? sqrt(a) : RC_F64_NAN;
+        case OP_TAN: { double t = tan(a); return isfinite(t) ? t : RC_F64_NAN; }
+        case OP_POW: return (a < 0.0 && fabs(b - round(b)) > 1e-9) ? RC_F64_NAN : pow(a, b);
+        default: return im_value(op, a, b);
+    }
+}
+
+__device__ __forceinline__ bool im_is_bin(uint32_t op)
+{
+    return op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV || op == OP_POW;
+}
+
+/* Signed distance (metric 0) or f itself (metric 1) of one row at constants
+   c; false when the row is undefined.  v / adj are this lane's node arrays. */
+__device__ __noinline__ bool im_distance(const uint32_t *__restrict__ r4, int m, const double *c,
+                            const double *__restrict__ cols64, int n, int nv, int row,
+                            int metric, double *v, double *adj, double &out)
+{
+    bool guard = false;
+    for (int i = 0; i < m; ++i) {
+        const uint32_t op = r4[4 * i], k0 = r4[4 * i + 1], k1 = r4[4 * i + 2], slot = r4[4 * i + 3];
+        double x;
+        if (op == OP_CONST) {
+            x = c[slot];
+        } else if (op == OP_VAR) {
+            x = cols64[(size_t)slot * (size_t)n + (size_t)row];
+        } else if (im_is_bin(op)) {
+            const double a = v[k0], b = v[k1];
+            x = im_value(op, a, b);
+            guard = guard || im_guard(op, a, b, x);
+        } else {
+            const double a = v[k0];
+            x = im_value(op, a, 0.0);
+            guard = guard || im_guard(op, a, 0.0, x);
+        }
+        v[i] = x;
+    }
+    double fv = v[m - 1];
+    if (guard) {
+        /* expr::eval over the same records, in adj[] as scratch. */
+        for (int i = 0; i < m; ++i) {
+            const uint32_t op = r4[4 * i], k0 = r4[4 * i + 1], k1 = r4[4 * i + 2];
+            if (op == OP_CONST || op == OP_VAR) adj[i] = v[i];
+            else if (im_is_bin(op)) adj[i] = im_guarded(op, adj[k0], adj[k1]);
+            else adj[i] = im_guarded(op, adj[k0], 0.0);
+        }
+        fv = adj[m - 1];
+    }
+    if (!isfinite(fv)) return false;
+    if (metric == 1) {
+        out = fv;
+        return true;
+    }
+    /* Adjoint sweep (backward_impl) with the input gradients as output. */
+    double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < m; ++i) adj[i] = 0.0;
+    adj[m - 1] = 1.0;
+    for (int i = m - 1; i >= 0; --i) {
+        const double a = adj[i];
+        if (a == 0.0) continue;
+        const uint32_t op = r4[4 * i], q0 = r4[4 * i + 1], q1 = r4[4 * i + 2], slot = r4[4 * i + 3];
+        if (op == OP_CONST) continue;
+        if (op == OP_VAR) {
+            if ((int)slot < nv) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if ((int)slot == q) g[q] += a;
+            }
+            continue;
+        }
+        const double x = v[q0];
+        switch (op) {
+            case OP_NEG: adj[q0] += -a; break;
+            case OP_EXP: adj[q0] += a * exp(x); break;
+            case OP_LOG: adj[q0] += a / x; break;
+            case OP_ABS: adj[q0] += a * (x >= 0.0 ? 1.0 : -1.0); break;
+            case OP_SQRT: adj[q0] += a / (2.0 * sqrt(x)); break;
+            case OP_SIN: adj[q0] += a * cos(x); break;
+            case OP_COS: adj[q0] += -a * sin(x); break;
+            case OP_TAN: { const double t = cos(x); adj[q0] += a / (t * t); } break;
+            case OP_SINH: adj[q0] += a * cosh(x); break;
+            case OP_COSH: adj[q0] += a * sinh(x); break;
+            case OP_TANH: { const double t = tanh(x); adj[q0] += a * (1.0 - t * t); } break;
+            case OP_SIG: {
+                const double s = 1.0 / (1.0 + exp(-x));
+                adj[q0] += a * s * (1.0 - s);
+            } break;
+            case OP_GAUSS: adj[q0] += a * (-2.0 * x) * exp(-x * x); break;
+            case OP_SQUARE: adj[q0] += a * 2.0 * x; break;
+            case OP_CUBE: adj[q0] += a * 3.0 * x * x; break;
+            case OP_ERF: adj[q0] += a * 2.0 * IM_INV_SQRT_PI * exp(-x * x); break;
+            case OP_ERFC: adj[q0] += -a * 2.0 * IM_INV_SQRT_PI * exp(-x * x); break;
+            case OP_ADD: adj[q0] += a; adj[q1] += a; break;
+            case OP_SUB: adj[q0] += a; adj[q1] += -a; break;
+            case OP_MUL: adj[q0] += a * v[q1]; adj[q1] += a * x; break;
+            case OP_DIV: {
+                const double d = v[q1];
+                adj[q0] += a / d;
+                adj[q1] += -a * x / (d * d);
+            } break;
+            case OP_POW: {
+                const double e = v[q1];
+                if (x > 0.0) {
+                    adj[q0] += a * e * pow(x, e - 1.0);
+                    adj[q1] += a * pow(x, e) * log(x);
+                }
+            } break;
+            default: break;
+        }
+    }
+    double norm_sq = 0.0;
+    for (int q = 0; q < 8; ++q) {
+        if (q >= nv) break;
+        if (!isfinite(g[q])) return false;
+        norm_sq += g[q] * g[q];
+    }
+    const double norm = sqrt(norm_sq);
+    if (!(isfinite(norm) && norm > 0.0)) return false;
+    const double d = fv / norm;
+    if (!isfinite(d)) return false;
+    out = d;
+    return true;
+}
+
+/* One pass over the work items.  cs holds, per candidate, the constant
+   vector of this pass (kmax stride).  part receives per-item partials
+   (pstride doubles), bad a per-item flag (1: a row with w > 0 is
+   undefined, 2: a weight is negative or not finite). */
+template <int MODE>
+__global__ __launch_bounds__(64) void im_pass_kernel(
+    const ImItem *__restrict__ items, int nitems, const ImCand *__restrict__ cands,
+    const uint32_t *__restrict__ rec, const double *__restrict__ cs, int kmax,
+    const double *__restrict__ cols64, const double *__restrict__ w64, int n, int nv, int metric,
+    double *__restrict__ part, int pstride, int *__restrict__ bad)
+{
+    const int it_i = blockIdx.x;
+    if (it_i >= nitems) return;
+    const ImItem it = items[it_i];
+    const ImCand d = cands[it.cand];
+    const int lane = threadIdx.x;
+    const uint32_t *r4 = rec + d.off;
+    const double *c = cs + (size_t)it.cand * (size_t)kmax;
+    const int k = d.k;
+    double v[IM_MAX_NODES], adj[IM_MAX_NODES];
+    double cset[IM_KMAX];
+    for (int j = 0; j < IM_KMAX; ++j) cset[j] = (j < k) ? c[j] : 0.0;
+    __shared__ double J[64][IM_KMAX + 2];
+    const int K1 = k + 1; /* [J | r] columns */
+    const int nstats = K1 * (K1 + 1) / 2;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0}; /* owned Gram entries (JAC) */
+    double s0 = 0.0, s1 = 0.0;
+    int flag = 0;
+    const int row_end = it.row0 + it.rows;
+    for (int base = it.row0; base < row_end; base += 64) {
+        const int row = base + lane;
+        const bool live = row < row_end;
+        const double w = (live && w64) ? w64[row] : 1.0;
+        if (live && (!isfinite(w) || w < 0.0)) flag |= 2;
+        const bool use = live && isfinite(w) && w > 0.0;
+        if (MODE == IM_SCORE || MODE == IM_RSS) {
+            if (use) {
+                double dist;
+                if (!im_distance(r4, d.m, cset, cols64, n, nv, row, MODE == IM_SCORE ? metric : 0,
+                                 v, adj, dist)) {
+                    flag |= 1;
+                } else if (MODE == IM_SCORE) {
+                    s0 += w * fabs(dist);
+                    s1 += w;
+                } else {
+                    const double r = sqrt(w) * dist;
+                    s0 += r * r;
+                }
+            }
+        } else {
+            double jrow[IM_KMAX + 1];
+            for (int j = 0; j <= IM_KMAX; ++j) jrow[j] = 0.0;
+            if (use) {
+                const double sw = sqrt(w);
+                double d0;
+                bool ok = im_distance(r4, d.m, cset, cols64, n, nv, row, 0, v, adj, d0);
+                jrow[k] = sw * d0;
+                for (int j = 0; j < k && ok; ++j) {
+                    const double cj = cset[j];
+                    const double h = 1e-6 * (1.0 + fabs(cj));
+                    double dp, dm;
+                    cset[j] = cj + h;
+                    ok = im_distance(r4, d.m, cset, cols64, n, nv, row, 0, v, adj, dp);
+                    if (ok) {
+                        cset[j] = cj - h;
+                        ok = im_distance(r4, d.m, cset, cols64, n, nv, row, 0, v, adj, dm);
+                    }
+                    cset[j] = cj;
+                    if (ok) jrow[j] = (sw * dp - sw * dm) / (2.0 * h);
+                }
+                if (!ok) flag |= 1;
+                for (int j = 0; j <= k; ++j)
+                    if (!isfinite(jrow[j])) flag |= 1;
+            }
+            for (int j = 0; j <= k; ++j) J[lane][j] = jrow[j];
+            __syncthreads();
+            for (int q = 0; q < 4; ++q) {
+                const int s = lane + 64 * q;
+                if (s >= nstats) break;
+                /* packed lower triangle: s = a (a + 1) / 2 + b */
+                int a = (int)((sqrt(8.0 * s + 1.0) - 1.0) * 0.5);
+                while (a * (a + 1) / 2 > s) --a;
+                while ((a + 1) * (a + 2) / 2 <= s) ++a;
+                const int b = s - a * (a + 1) / 2;
+                double t = acc[q];
+                for (int r = 0; r < 64; ++r) t += J[r][a] * J[r][b];
+                acc[q] = t;
+            }
+            __syncthreads();
+        }
+    }
+    double *p = part + (size_t)it_i * (size_t)pstride;
+    int f = flag;
+    for (int off = 32; off > 0; off >>= 1) f |= __shfl_xor(f, off, 64);
+    if (MODE == IM_JAC) {
+        for (int q = 0; q < 4; ++q) {
+            const int s = lane + 64 * q;
+            if (s < nstats) p[s] = acc[q];
+        }
+    } else {
+        const double t0 = rc_wave_sum_f64(s0);
+        const double t1 = rc_wave_sum_f64(s1);
+        if (lane == 0) {
+            p[0] = t0;
+            p[1] = t1;
+        }
+    }
+    if (lane == 0) bad[it_i] = f;
+}
+
+/* implicit.rs:215-253: Gaussian elimination with partial pivoting on the
+   damped normal equations. */
+bool im_solve(const std::vector<double> &A, const std::vector<double> &b, double lambda, int k,
+              std::vector<double> &d)
+{
+    std::vector<std::vector<double>> m((size_t)k, std::vector<double>((size_t)k + 1, 0.0));
+    for (int i = 0; i < k; ++i) {
+        for (int j = 0; j < k; ++j) m[i][j] = A[(size_t)i * k + j];
+        m[i][i] = m[i][i] * (1.0 + lambda) + 1e-12;
+        m[i][k] = b[i];
+    }
+    for (int col = 0; col < k; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < k; ++r)
+            if (fabs(m[r][col]) > fabs(m[piv][col])) piv = r;
+        if (fabs(m[piv][col]) < 1e-300) return false;
+        std::swap(m[col], m[piv]);
+        for (int r = col + 1; r < k; ++r) {
+            const double factor = m[r][col] / m[col][col];
+            if (factor != 0.0)
+                for (int cc = col; cc <= k; ++cc) m[r][cc] -= factor * m[col][cc];
+        }
+    }
+    d.assign((size_t)k, 0.0);
+    for (int i = k - 1; i >= 0; --i) {
+        double s = m[i][k];
+        for (int j = i + 1; j < k; ++j) s -= m[i][j] * d[j];
+        d[i] = s / m[i][i];
+    }
+    return true;
+}
+
+/* Host side: validated candidates, resident buffers, pass launcher. */
+struct ImBatch {
+    RcCtx *c = nullptr;
+    int nprog = 0, kmax = 0;
+    std::vector<ImCand> cands;
+    std::vector<int> pre; /* -1 device, else a host status */
+    std::vector<uint32_t> rec;
+    uint32_t *d_rec = nullptr;
+    ImCand *d_cand = nullptr;
+    ImItem *d_items = nullptr;
+    double *d_cs = nullptr, *d_part = nullptr;
+    int *d_bad = nullptr;
+    std::vector<ImItem> h_items;
+    std::vector<double> h_part;
+    std::vector<int> h_bad;
+    float ms = 0.0f;
+
+    ~ImBatch()
+    {
+        if (d_rec) (void)hipFree(d_rec);
+        if (d_cand) (void)hipFree(d_cand);
+        if (d_items) (void)hipFree(d_items);
+        if (d_cs) (void)hipFree(d_cs);
+        if (d_part) (void)hipFree(d_part);
+        if (d_bad) (void)hipFree(d_bad);
+    }
+
+    /* Post-order validation as the fit path's (canonical children, supported
+       operators, slots in range). */
+    int setup(RcCtx *ctx, const unsigned *prog, const unsigned *off, const int *pcount, int np,
+              int km)
+    {
+        c = ctx;
+        nprog = np;
+        kmax = km;
+        RcRows &R = c->sets[0];
+        cands.assign((size_t)np, ImCand{0, 0, 0, 0});
+        pre.assign((size_t)np, -1);
+        for (int i = 0; i < np; ++i) {
+            const int words = (int)(off[i + 1] - off[i]);
+            const int m = words / 4;
+            const unsigned *w = prog + off[i];
+            const int p = pcount[i];
+            bool ok = words % 4 == 0 && m >= 1 && m <= IM_MAX_NODES && p >= 0 && p <= kmax &&
+                      p <= IM_KMAX;
+            std::vector<int> st;
+            for (int q = 0; ok && q < m; ++q) {
+                const uint32_t op = w[4 * q], k0 = w[4 * q + 1], k1 = w[4 * q + 2], slot = w[4 * q + 3];
+                if (op == OP_CONST) {
+                    ok = (int)slot < p;
+                    st.push_back(q);
+                } else if (op == OP_VAR) {
+                    ok = (int)slot < R.v && slot < 8u;
+                    st.push_back(q);
+                } else if (op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV ||
+                           op == OP_POW) {
+                    ok = st.size() >= 2 && st.back() == (int)k1 && st[st.size() - 2] == (int)k0;
+                    if (ok) { st.pop_back(); st.pop_back(); st.push_back(q); }
+                } else {
+                    const bool known = op == OP_SIN || op == OP_COS || op == OP_TAN || op == OP_EXP ||
+                                       op == OP_LOG || op == OP_SQRT || op == OP_ABS ||
+                                       op == OP_TANH || op == OP_SIG || op == OP_SQUARE ||
+                                       op == OP_CUBE || op == OP_NEG || op == OP_GAUSS ||
+                                       op == OP_SINH || op == OP_COSH || op == OP_ERF ||
+                                       op == OP_ERFC;
+                    ok = known && !st.empty() && st.back() == (int)k0;
+                    if (ok) { st.pop_back(); st.push_back(q); }
+                }
+            }
+            ok = ok && st.size() == 1 && st[0] == m - 1;
+            if (!ok) {
+                pre[i] = 1;
+                continue;
+            }
+            cands[i].off = (uint32_t)rec.size();
+            cands[i].m = m;
+            cands[i].k = p;
+            rec.insert(rec.end(), w, w + words);
+        }
+        if (rec.empty()) rec.push_back(0);
+        for (int i = 0; i < np; ++i) {
+            if (pre[i] >= 0) continue;
+            for (int r0 = 0; r0 < R.n; r0 += IM_TILE)
+                h_items.push_back(ImItem{i, r0, std::min(IM_TILE, R.n - r0), 0});
+        }
+        const size_t ni = std::max<size_t>(1, h_items.size());
+        if (hipMalloc((void **)&d_rec, rec.size() * 4) != hipSuccess ||
+            hipMalloc((void **)&d_cand, sizeof(ImCand) * (size_t)np) != hipSuccess ||
+            hipMalloc((void **)&d_items, sizeof(ImItem) * ni) != hipSuccess ||
+            hipMalloc((void **)&d_cs, sizeof(double) * (size_t)np * (size_t)kmax) != hipSuccess ||
+            hipMalloc((void **)&d_part, sizeof(double) * ni * IM_STATS) != hipSuccess ||
+            hipMalloc((void **)&d_bad, sizeof(int) * ni) != hipSuccess) {
+            (void)hipGetLastError();
+            return -1;
+        }
+        hipStream_t s = c->s_fit;
+        if (hipMemcpyAsync(d_rec, rec.data(), rec.size() * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(d_cand, cands.data(), sizeof(ImCand) * (size_t)np, hipMemcpyHostToDevice, s) != hipSuccess ||
+            (!h_items.empty() && hipMemcpyAsync(d_items, h_items.data(), sizeof(ImItem) * h_items.size(),
+                                                hipMemcpyHostToDevice, s) != hipSuccess))
+            return -1;
+        h_part.resize(ni * IM_STATS);
+        h_bad.resize(ni);
+        return 0;
+    }
+
+    /* Runs one pass over the items of the candidates with run[i] != 0 at
+       constants cs (np x kmax); fills per-candidate sums: out[i][0..] and the
+       OR of the item flags. */
+    int pass(int mode, int metric, const std::vector<double> &cs, const std::vector<char> &run,
+             std::vector<std::vector<double>> &sums, std::vector<int> &flags)
+    {
+        RcRows &R = c->sets[0];
+        std::vector<ImItem> sel;
+        std::vector<int> from;
+        for (size_t q = 0; q < h_items.size(); ++q)
+            if (run[(size_t)h_items[q].cand]) {
+                sel.push_back(h_items[q]);
+                from.push_back((int)q);
+            }
+        sums.assign((size_t)nprog, std::vector<double>());
+        flags.assign((size_t)nprog, 0);
+        if (sel.empty()) return 0;
+        hipStream_t s = c->s_fit;
+        if (hipMemcpyAsync(d_items, sel.data(), sizeof(ImItem) * sel.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(d_cs, cs.data(), sizeof(double) * cs.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+            return -1;
+        const int ni = (int)sel.size();
+        (void)hipEventRecord(c->t0, s);
+        if (mode == IM_SCORE)
+            im_pass_kernel<IM_SCORE><<<ni, 64, 0, s>>>(d_items, ni, d_cand, d_rec, d_cs, kmax, R.cols64,
+                                                        R.w64, R.n, R.v, metric, d_part, IM_STATS, d_bad);
+        else if (mode == IM_RSS)
+            im_pass_kernel<IM_RSS><<<ni, 64, 0, s>>>(d_items, ni, d_cand, d_rec, d_cs, kmax, R.cols64,
+                                                      R.w64, R.n, R.v, metric, d_part, IM_STATS, d_bad);
+        else
+            im_pass_kernel<IM_JAC><<<ni, 64, 0, s>>>(d_items, ni, d_cand, d_rec, d_cs, kmax, R.cols64,
+                                                      R.w64, R.n, R.v, metric, d_part, IM_STATS, d_bad);
+        (void)hipEventRecord(c->t1, s);
+        if (hipGetLastError() != hipSuccess) return -1;
+        if (hipMemcpyAsync(h_part.data(), d_part, sizeof(double) * (size_t)ni * IM_STATS,
+                           hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(h_bad.data(), d_bad, sizeof(int) * (size_t)ni, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return -1;
+        float e = 0.0f;
+        (void)hipEventElapsedTime(&e, c->t0, c->t1);
+        ms += e;
+        /* Items of one candidate are consecutive and in row order. */
+        for (int q = 0; q < ni; ++q) {
+            const int cand = sel[(size_t)q].cand;
+            const int K1 = cands[(size_t)cand].k + 1;
+            const int nst = (mode == IM_JAC) ? K1 * (K1 + 1) / 2 : 2;
+            std::vector<double> &t = sums[(size_t)cand];
+            if (t.empty()) t.assign((size_t)nst, 0.0);
+            for (int j = 0; j < nst; ++j) t[(size_t)j] += h_part[(size_t)q * IM_STATS + j];
+            flags[(size_t)cand] |= h_bad[(size_t)q];
+        }
+        return 0;
+    }
+};
+
+constexpr double IM_INF = __builtin_inf();
+
+} /* namespace */
+
+int rc_rows_ensure_f64(RcCtx *c, RcRows *r);
+
+extern "C" {
+
+float rc_implicit_score_batch(void *ctx, const unsigned int *prog, const unsigned int *off,
+                              const int *pcount, int nprog, int kmax, const double *c, int metric,
+                              double *out, int *status)
+{
+    RcCtx *cx = (RcCtx *)ctx;
+    if (!cx || !prog || !off || !pcount || !c || !out || !status || nprog <= 0 || kmax <= 0 ||
+        kmax > IM_KMAX || (metric != 0 && metric != 1))
+        return -1.0f;
+    if (rc_rows_ensure_f64(cx, &cx->sets[0]) != 0) return -1.0f;
+    ImBatch B;
+    if (B.setup(cx, prog, off, pcount, nprog, kmax) != 0) return -1.0f;
+    std::vector<double> cs(c, c + (size_t)nprog * (size_t)kmax);
+    std::vector<char> run((size_t)nprog, 1);
+    for (int i = 0; i < nprog; ++i) run[i] = B.pre[i] < 0;
+    std::vector<std::vector<double>> sums;
+    std::vector<int> flags;
+    if (B.pass(IM_SCORE, metric, cs, run, sums, flags) != 0) return -1.0f;
+    for (int i = 0; i < nprog; ++i) {
+        if (B.pre[i] >= 0) {
+            out[i] = RC_F64_NAN;
+            status[i] = 1;
+            continue;
+        }
+        const std::vector<double> &t = sums[(size_t)i];
+        const bool fin = !t.empty() && flags[i] == 0 && isfinite(t[1]) && t[1] > 0.0;
+        out[i] = fin ? t[0] / t[1] : IM_INF;
+        status[i] = fin ? 0 : 2;
+    }
+    return B.ms;
+}
+
+float rc_implicit_fit_batch(void *ctx, const unsigned int *prog, const unsigned int *off,
+                            const int *pcount, int nprog, int kmax, const double *c0, int iters,
+                            double *outc, double *outsse, int *outstatus)
+{
+    RcCtx *cx = (RcCtx *)ctx;
+    if (!cx || !prog || !off || !pcount || !c0 || !outc || !outsse || !outstatus || nprog <= 0 ||
+        kmax <= 0 || kmax > IM_KMAX || iters < 0)
+        return -1.0f;
+    if (rc_rows_ensure_f64(cx, &cx->sets[0]) != 0) return -1.0f;
+    ImBatch B;
+    if (B.setup(cx, prog, off, pcount, nprog, kmax) != 0) return -1.0f;
+    const size_t P = (size_t)nprog, KM = (size_t)kmax;
+    std::vector<double> c(c0, c0 + P * KM), best_c = c, cs = c;
+    std::vector<double> best(P, IM_INF), base(P, IM_INF), lambda(P, 1e-3);
+    std::vector<char> active(P, 0), want(P, 0);
+    std::vector<std::vector<double>> sums;
+    std::vector<int> flags;
+    /* Residual cost at the start (rss_of). */
+    for (size_t i = 0; i < P; ++i) want[i] = B.pre[i] < 0;
+    if (B.pass(IM_RSS, 0, c, want, sums, flags) != 0) return -1.0f;
+    for (size_t i = 0; i < P; ++i) {
+        if (!want[i]) continue;
+        const bool ok = flags[i] == 0 && !sums[i].empty() && isfinite(sums[i][0]);
+        base[i] = ok ? sums[i][0] : IM_INF;
+        best[i] = base[i];
+        /* no constants or no iterations: the start is the answer */
+        active[i] = ok && B.cands[i].k > 0 && iters > 0;
+    }
+    for (int it = 0; it < std::max(iters, 1); ++it) {
+        bool any = false;
+        for (size_t i = 0; i < P; ++i) any = any || active[i];
+        if (!any) break;
+        if (B.pass(IM_JAC, 0, c, active, sums, flags) != 0) return -1.0f;
+        std::fill(want.begin(), want.end(), 0);
+        for (size_t i = 0; i < P; ++i) {
+            if (!active[i]) continue;
+            const int k = B.cands[i].k, K1 = k + 1;
+            const std::vector<double> &t = sums[i];
+            if (flags[i] != 0 || t.empty()) { active[i] = 0; continue; }
+            std::vector<double> jtj((size_t)k * k), rhs((size_t)k);
+            bool fin = true;
+            for (int a = 0; a < k; ++a) {
+                for (int b = 0; b <= a; ++b) {
+                    const double v = t[(size_t)(a * (a + 1) / 2 + b)];
+                    jtj[(size_t)a * k + b] = v;
+                    jtj[(size_t)b * k + a] = v;
+                    fin = fin && isfinite(v);
+                }
+                rhs[a] = -t[(size_t)(k * (k + 1) / 2 + a)];
+                fin = fin && isfinite(rhs[a]);
+            }
+            (void)K1;
+            if (!fin) { active[i] = 0; continue; }
+            std::vector<double> step;
+            if (!im_solve(jtj, rhs, lambda[i], k, step)) {
+                lambda[i] *= 6.0;
+                if (lambda[i] > 1e9) active[i] = 0;
+                continue;
+            }
+            for (int a = 0; a < k; ++a)
+                cs[i * KM + a] = std::min(1e6, std::max(-1e6, c[i * KM + a] + step[a]));
+            want[i] = 1;
+        }
+        bool trial = false;
+        for (size_t i = 0; i < P; ++i) trial = trial || want[i];
+        if (!trial) continue;
+        if (B.pass(IM_RSS, 0, cs, want, sums, flags) != 0) return -1.0f;
+        for (size_t i = 0; i < P; ++i) {
+            if (!want[i]) continue;
+            const bool ok = flags[i] == 0 && !sums[i].empty();
+            const double s = ok ? sums[i][0] : IM_INF;
+            if (ok && s < best[i]) {
+                best[i] = s;
+                for (size_t a = 0; a < KM; ++a) c[i * KM + a] = best_c[i * KM + a] = cs[i * KM + a];
+                lambda[i] = std::max(lambda[i] / 8.0, 1e-12);
+                if (best[i] == 0.0) active[i] = 0;
+            } else {
+                lambda[i] *= 6.0;
+                if (lambda[i] > 1e9) active[i] = 0;
+            }
+        }
+    }
+    for (size_t i = 0; i < P; ++i) {
+        for (size_t a = 0; a < KM; ++a) outc[i * KM + a] = c0[i * KM + a];
+        if (B.pre[i] >= 0) {
+            outsse[i] = RC_F64_NAN;
+            outstatus[i] = 1;
+            continue;
+        }
+        if (!isfinite(base[i])) {
+            outsse[i] = IM_INF;
+            outstatus[i] = 7;
+            continue;
+        }
+        if (best[i] < base[i])
+            for (size_t a = 0; a < KM; ++a) outc[i * KM + a] = best_c[i * KM + a];
+        outsse[i] = best[i];
+        outstatus[i] = 0;
+    }
+    return B.ms;
+}
+
+} /* extern "C" */

@@ -88,8 +88,8 @@ SIGNATURES = {
     "rc_unified": (C.c_int, []),
     "rch_compile_program": (C.c_int, [i32p, f64p, C.c_int, u32p, C.c_int, i32p]),
     "rch_fit_plan": (C.c_int, [i32p, f64p, C.c_int, u32p, C.c_int, f64p, C.c_int, i32p]),
-    "rch_fitvm_compile": (C.c_int, [u32p, C.c_int, C.c_int, C.c_int, C.c_int, u32p, C.c_int, u32p,
-                                    C.c_int, i32p]),
+    "rch_fitvm_compile": (C.c_int, [u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, u32p,
+                                    C.c_int, u32p, C.c_int, i32p]),
 }
 
 _lib = None
