@@ -38,6 +38,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def copy_probe():
+    """A plain 256 MiB device-to-device copy, three times: under rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE it gives the counters' reading for a known
+    number of HBM bytes (256 MiB read and 256 MiB written per dispatch)."""
+    import torch
+    nbytes = 256 << 20
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes,
+                      "dispatches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,7 +67,12 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a device copy of 256 MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this setup has one MI355X; "
               f"--gpus {args.gpus} is not supported (candidate sharding needs no "

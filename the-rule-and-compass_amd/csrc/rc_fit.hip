@@ -29,7 +29,8 @@
  * a hipGraph and replayed.
  */
 #include "rc_fit_dev.h"
-#include "rc_fitvm_dev.h"
+#include "rc_fitvm.h"
+#include "rc_fitvm_kern.h"
 #include "rc_internal.h"
 
 #include <math.h>
@@ -521,248 +522,6 @@ __global__ __launch_bounds__(64) void ft_trial_kernel(
         ft_value_rows<T, S, R>(fc, d.t_m, cst, cols, R_.n, xrow, pred);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const double wb = R_.w64 ? R_.w64[xrow[r]] : 1.0;
-            if (!live[r] || !(wb > 0.0)) continue;
-            const double rr = (double)pred[r] - R_.y64[xrow[r]];
-            bool hard = !isfinite(pred[r]) || !isfinite(rr);
-            if (sizeof(T) == 4) hard = hard || fabs((double)pred[r]) > 1e30;
-            lsum += hard ? 1e30 : wb * rc_row_rho(rr, d.metric);
-        }
-    }
-    const double ls = rc_wave_sum_f64(lsum);
-    if (lane == 0) ploss[item] = ls;
-}
-
-/* ---- fit VM kernels: two rows per lane ----------------------------------- */
-
-constexpr int XS2 = 130; /* LDS column stride of the 128-row augmented block */
-
-/* Per-row residual bookkeeping shared by the assembly kernels. */
-struct RowPrep {
-    double r64, s64, wb;
-    bool live, ok;
-};
-
-__device__ __forceinline__ RowPrep vm_row_prep(bool live, double wb, float pred, double p64,
-                                               double y64, int metric, bool polish)
-{
-    RowPrep q;
-    q.live = live;
-    q.wb = wb;
-    q.r64 = p64 - y64;
-    q.ok = live && isfinite(p64) && (polish || fabsf(pred) <= 1e30f) && isfinite(q.r64);
-    q.s64 = sqrt(rc_row_weight(wb, q.r64, metric));
-    return q;
-}
-
-/* Assembly on the fit VM: the same per-row contract as ft_asm_kernel (the
-   augmented row [s*J | s*r_hi | s*r_lo] of every row into the Gram block,
-   the loss sum, bad-row and skipped-row flags), 128 rows per wave step. */
-template <int KC, bool POLISH>
-__global__ __launch_bounds__(64) void vm_asm_kernel(
-    const FtItem *__restrict__ items, const int *__restrict__ list, int nlist,
-    const FtCand *__restrict__ cands, const FtRows *__restrict__ rowsd,
-    const uint32_t *__restrict__ code, const int *__restrict__ run, const float *__restrict__ cp32,
-    const double *__restrict__ cp64, double *__restrict__ pgram, int tri_stride,
-    double *__restrict__ ploss, int *__restrict__ pbad, int *__restrict__ pskip)
-{
-    const int li = blockIdx.x;
-    if (li >= nlist) return;
-    const int item = list[li];
-    const FtItem it = items[item];
-    const int cand = it.cand;
-    if (!run[cand]) return;
-    const FtCand d = cands[cand];
-    const FtRows R = rowsd[d.rowset];
-    const int lane = threadIdx.x;
-    const int k = d.k;
-
-    __shared__ float X[KC * XS2];
-    for (int i = lane; i < KC * XS2; i += 64) X[i] = 0.0f;
-    __syncthreads();
-
-    const float *cst = cp32 + (size_t)cand * FT_KMAX;
-    const double *cst64 = cp64 + (size_t)cand * FT_KMAX;
-    const uint2 *fc = (const uint2 *)(code + d.fwd_off);
-    const uint2 *rc = (const uint2 *)(code + d.rev_off);
-    char *xl = (char *)(X + lane);
-    floatx4 acc00 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
-    double g00[4] = {0, 0, 0, 0}, g10[4] = {0, 0, 0, 0}, g11[4] = {0, 0, 0, 0};
-    double lsum = 0.0;
-    int bad = 0, skip = 0;
-    vm_u64x16 U = (vm_u64x16)(0), B = (vm_u64x16)(0), U_ = U, B_ = B;
-    vm_u64x8 XV = (vm_u64x8)(0), XV_ = XV;
-    const int row_end = it.row0 + it.rows;
-    int step = 0;
-    for (int base = it.row0; base < row_end; base += 128, ++step) {
-        const int r0 = base + lane, r1 = base + 64 + lane;
-        const bool in0 = r0 < row_end, in1 = r1 < row_end;
-        const int q0 = in0 ? r0 : it.row0, q1 = in1 ? r1 : it.row0;
-        const double wb0 = R.w64 ? R.w64[q0] : 1.0, wb1 = R.w64 ? R.w64[q1] : 1.0;
-#pragma unroll
-        for (int v = 0; v < VM_MAX_VARS; ++v)
-            if (v < R.v) {
-                const float *col = R.cols + (size_t)v * (size_t)R.n;
-                VmPack<vm_f2>::set(XV, XV_, XV_, XV_, v, (vm_f2){col[q0], col[q1]});
-            }
-        const vm_f2 pred = vm_forward<vm_f2, true>(fc, d.m, cst, XV, XV_, XV_, XV_, U, U_, U_, U_, B,
-                                                   B_, B_, B_);
-        double p0 = (double)pred.x, p1 = (double)pred.y;
-        if (POLISH) {
-            dv16 Ua = (dv16)(0.0), Ub = Ua, Ba = Ua, Bb = Ua;
-            dv8 Xa = (dv8)(0.0), Xb = Xa;
-#pragma unroll
-            for (int v = 0; v < VM_MAX_VARS; ++v)
-                if (v < R.v) {
-                    const double *col = R.cols64 + (size_t)v * (size_t)R.n;
-                    VmPack<vm_d2>::set(Xa, Xb, Xb, Xb, v, (vm_d2){col[q0], col[q1]});
-                }
-            const vm_d2 pp = vm_forward<vm_d2, false>(fc, d.m, cst64, Xa, Xb, Xb, Xb, Ua, Ub, Ub, Ub,
-                                                      Ba, Bb, Bb, Bb);
-            p0 = pp.x;
-            p1 = pp.y;
-        }
-        const RowPrep a0 = vm_row_prep(in0 && wb0 > 0.0, wb0, pred.x, p0, R.y64[q0], d.metric, POLISH);
-        const RowPrep a1 = vm_row_prep(in1 && wb1 > 0.0, wb1, pred.y, p1, R.y64[q1], d.metric, POLISH);
-        const vm_f2 seed = {a0.ok ? (float)a0.s64 : 0.0f, a1.ok ? (float)a1.s64 : 0.0f};
-        const vm_f2 fin = vm_reverse(rc, d.nrev, seed, cst, XV, U, B, xl);
-        const bool gf0 = fin.x == 0.0f, gf1 = fin.y == 0.0f;
-        const bool use0 = a0.ok && gf0, use1 = a1.ok && gf1;
-        if (POLISH) {
-            if ((a0.live && !a0.ok) || (a1.live && !a1.ok)) bad = 1;
-            skip += (a0.ok && !gf0 ? 1 : 0) + (a1.ok && !gf1 ? 1 : 0);
-            if (a0.ok) lsum += a0.wb * rc_row_rho(a0.r64, d.metric);
-            if (a1.ok) lsum += a1.wb * rc_row_rho(a1.r64, d.metric);
-        } else {
-            if ((a0.live && !use0) || (a1.live && !use1)) bad = 1;
-            if (use0) lsum += a0.wb * rc_row_rho(a0.r64, d.metric);
-            if (use1) lsum += a1.wb * rc_row_rho(a1.r64, d.metric);
-        }
-        if (__any((a0.ok && !gf0) || (a1.ok && !gf1))) {
-            for (int j = 0; j < k; ++j) {
-                if (!use0) X[j * XS2 + lane] = 0.0f;
-                if (!use1) X[j * XS2 + 64 + lane] = 0.0f;
-            }
-        }
-        const double rs0 = a0.s64 * a0.r64, rs1 = a1.s64 * a1.r64;
-        const float rh0 = (float)rs0, rh1 = (float)rs1;
-        X[k * XS2 + lane] = use0 ? rh0 : 0.0f;
-        X[(k + 1) * XS2 + lane] = use0 ? (float)(rs0 - (double)rh0) : 0.0f;
-        X[k * XS2 + 64 + lane] = use1 ? rh1 : 0.0f;
-        X[(k + 1) * XS2 + 64 + lane] = use1 ? (float)(rs1 - (double)rh1) : 0.0f;
-        __syncthreads();
-        const int col = lane & 15;
-        const int sub = lane >> 4;
-#pragma unroll 4
-        for (int q = 0; q < 32; ++q) {
-            const float b0 = X[col * XS2 + 4 * q + sub];
-            acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, b0, acc00, 0, 0, 0);
-            if (KC == 32) {
-                const float b1 = X[(16 + col) * XS2 + 4 * q + sub];
-                acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, b0, acc10, 0, 0, 0);
-                acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, b1, acc11, 0, 0, 0);
-            }
-        }
-        __syncthreads();
-        if (step & 1) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                g00[r] += (double)acc00[r];
-                acc00[r] = 0.0f;
-                if (KC == 32) {
-                    g10[r] += (double)acc10[r];
-                    g11[r] += (double)acc11[r];
-                    acc10[r] = 0.0f;
-                    acc11[r] = 0.0f;
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        g00[r] += (double)acc00[r];
-        if (KC == 32) {
-            g10[r] += (double)acc10[r];
-            g11[r] += (double)acc11[r];
-        }
-    }
-    const int K = k + 2;
-    double *pg = pgram + (size_t)item * (size_t)tri_stride;
-    const int j = lane & 15;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int i = 4 * (lane >> 4) + r;
-        if (i >= j && i < K) pg[i * (i + 1) / 2 + j] = g00[r];
-        if (KC == 32) {
-            const int i1 = 16 + i;
-            if (i1 < K) pg[i1 * (i1 + 1) / 2 + j] = g10[r];
-            if (i >= j && i1 < K) pg[i1 * (i1 + 1) / 2 + 16 + j] = g11[r];
-        }
-    }
-    const double ls = rc_wave_sum_f64(lsum);
-    int sk = skip;
-    for (int off = 32; off > 0; off >>= 1) sk += __shfl_xor(sk, off, 64);
-    const int anybad = __any(bad) ? 1 : 0;
-    if (lane == 0) {
-        ploss[item] = ls;
-        pbad[item] = anybad;
-        pskip[item] = sk;
-    }
-}
-
-/* Trial / exact loss on the fit VM: the value-pass stream (inputs in U
-   slots 0..v-1, no tape), 8 f32 rows or 4 f64 rows per lane, with an 8-slot
-   U file of four 64-bit-element vectors. */
-template <typename V>
-__global__ __launch_bounds__(64) void vm_trial_kernel(
-    const FtItem *__restrict__ items, const int *__restrict__ list, int nlist,
-    const FtCand *__restrict__ cands, const FtRows *__restrict__ rowsd,
-    const uint32_t *__restrict__ code, const int *__restrict__ run,
-    const typename VmScalar<V>::T *__restrict__ cp, double *__restrict__ ploss)
-{
-    typedef typename VmScalar<V>::T T;
-    typedef VmPack<V> P;
-    typedef typename P::F8 F;
-    constexpr int RR = VmScalar<V>::R;
-    const int li = blockIdx.x;
-    if (li >= nlist) return;
-    const int item = list[li];
-    const FtItem it = items[item];
-    const int cand = it.cand;
-    if (!run[cand]) return;
-    const FtCand d = cands[cand];
-    const FtRows R_ = rowsd[d.rowset];
-    const int lane = threadIdx.x;
-    const T *cst = cp + (size_t)cand * FT_KMAX;
-    const uint2 *fc = (const uint2 *)(code + d.t_off);
-    const T *cols;
-    if (sizeof(T) == 4) cols = (const T *)(const void *)R_.cols;
-    else cols = (const T *)(const void *)R_.cols64;
-    F U0 = (F)(0), U1 = U0, U2 = U0, U3 = U0, Bd = U0;
-    double lsum = 0.0;
-    const int row_end = it.row0 + it.rows;
-    for (int base = it.row0; base < row_end; base += 64 * RR) {
-        int xrow[RR];
-        bool live[RR];
-#pragma unroll
-        for (int r = 0; r < RR; ++r) {
-            const int row = base + r * 64 + lane;
-            live[r] = row < row_end;
-            xrow[r] = live[r] ? row : it.row0;
-        }
-#pragma unroll
-        for (int v = 0; v < VM_MAX_VARS; ++v)
-            if (v < R_.v) {
-                const T *col = cols + (size_t)v * (size_t)R_.n;
-                V xv;
-#pragma unroll
-                for (int r = 0; r < RR; ++r) xv[r] = col[xrow[r]];
-                P::set(U0, U1, U2, U3, v, xv);
-            }
-        const V pred = vm_forward<V, false>(fc, d.t_m, cst, U0, U1, U2, U3, U0, U1, U2, U3, Bd, Bd,
-                                            Bd, Bd);
-#pragma unroll
-        for (int r = 0; r < RR; ++r) {
             const double wb = R_.w64 ? R_.w64[xrow[r]] : 1.0;
             if (!live[r] || !(wb > 0.0)) continue;
             const double rr = (double)pred[r] - R_.y64[xrow[r]];
@@ -1375,10 +1134,11 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     std::vector<uint32_t> code;
     std::vector<FtItem> items;
     std::vector<int> pre((size_t)nprog, -1);
-    /* Assembly classes: fit VM (KC16, KC32), general interpreter with the
+    /* Assembly classes: fit VM with 4 rows per lane (KC16, KC32), fit VM
+       with 2 rows per lane (KC16, KC32), general interpreter with the
        register stores (KC16, KC32) and with the oversized stores (KC16,
        KC32).  Value-pass classes: fit VM, general small, general big. */
-    enum { NA = 6, NT = 3 };
+    enum { NA = 8, NT = 3 };
     std::vector<int> list[NA];
     std::vector<int> tlist[NT];
     int maxK = 2;
@@ -1395,9 +1155,11 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         const int words = (int)(off[i + 1] - off[i]);
         int info[6] = {0, 0, 0, 0, 0, 0};
         const int nv = c->sets[d.rowset].v;
-        const int vmrc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, XS2 * 4, 0u,
-                                           vf.data(), (int)vf.size(), vr.data(), (int)vr.size(),
-                                           info);
+        const int vmrc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, VM_XS * 4,
+                                           VMC_INPUTS_IN_U, vf.data(),
+                                           (int)vf.size(), vr.data(), (int)vr.size(), info);
+        /* 4 rows per lane: an 8-slot B file and one saved adjoint. */
+        const bool vm4 = vmrc == 0 && info[3] <= 8 && info[4] <= 1;
         if (vmrc == 0) {
             if (code.size() & 1) code.push_back(0);
             d.vm = 1;
@@ -1418,7 +1180,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         /* The value-pass stream: the fit VM's value-only form (inputs in U,
            8-slot file) when it fits, else the general interpreter's. */
         int tinfo[6] = {0, 0, 0, 0, 0, 0};
-        const int vorc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, XS2 * 4,
+        const int vorc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, VM_XS * 4,
                                            VMC_INPUTS_IN_U | VMC_VALUE_ONLY | VMC_SMALL_U,
                                            vf.data(), (int)vf.size(), nullptr, 0, tinfo);
         int tcls;
@@ -1438,7 +1200,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         d.item0 = (int)items.size();
         const bool big = prog_is_big(progs[i]);
         const int kc = d.k + 2 <= 16 ? 0 : 1;
-        const int cls = d.vm ? kc : (big ? 4 : 2) + kc;
+        const int cls = d.vm ? (vm4 ? 0 : 2) + kc : (big ? 6 : 4) + kc;
         maxK = std::max(maxK, d.k + 2);
         for (int r0 = 0; r0 < n; r0 += TILE_ROWS) {
             FtItem it;
@@ -1572,21 +1334,31 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         kern<<<nl, 64, 0, s>>>(d_items, d_lists + lofs[q], nl, d_cand, d_rows, d_code, d_run_a,
                                d_cp32, d_cp64, d_pgram, tri_stride, d_ploss, d_pbad, d_pskip);
     };
+    auto vm_asm = [&](int kc, int r, bool polish, int q) {
+        vm_launch_asm(kc, r, polish, lofs[q + 1] - lofs[q], s, d_items, d_lists + lofs[q], d_cand,
+                      d_rows, d_code, d_run_a, d_cp32, d_cp64, d_pgram, tri_stride, d_ploss,
+                      d_pbad, d_pskip);
+    };
     auto asm_pass0 = [&](int polish) {
         if (polish) {
-            asm_launch(vm_asm_kernel<16, true>, 0);
-            asm_launch(vm_asm_kernel<32, true>, 1);
-            asm_launch(ft_asm_kernel<16, false, true>, 2);
-            asm_launch(ft_asm_kernel<32, false, true>, 3);
-            asm_launch(ft_asm_kernel<16, true, true>, 4);
-            asm_launch(ft_asm_kernel<32, true, true>, 5);
+            /* the f64 polish residual runs two rows per lane for every VM class */
+            vm_asm(16, 2, true, 0);
+            vm_asm(32, 2, true, 1);
+            vm_asm(16, 2, true, 2);
+            vm_asm(32, 2, true, 3);
+            asm_launch(ft_asm_kernel<16, false, true>, 4);
+            asm_launch(ft_asm_kernel<32, false, true>, 5);
+            asm_launch(ft_asm_kernel<16, true, true>, 6);
+            asm_launch(ft_asm_kernel<32, true, true>, 7);
         } else {
-            asm_launch(vm_asm_kernel<16, false>, 0);
-            asm_launch(vm_asm_kernel<32, false>, 1);
-            asm_launch(ft_asm_kernel<16, false, false>, 2);
-            asm_launch(ft_asm_kernel<32, false, false>, 3);
-            asm_launch(ft_asm_kernel<16, true, false>, 4);
-            asm_launch(ft_asm_kernel<32, true, false>, 5);
+            vm_asm(16, 4, false, 0);
+            vm_asm(32, 4, false, 1);
+            vm_asm(16, 2, false, 2);
+            vm_asm(32, 2, false, 3);
+            asm_launch(ft_asm_kernel<16, false, false>, 4);
+            asm_launch(ft_asm_kernel<32, false, false>, 5);
+            asm_launch(ft_asm_kernel<16, true, false>, 6);
+            asm_launch(ft_asm_kernel<32, true, false>, 7);
         }
     };
     auto asm_pass = [&](int polish) { mark(0, [&]() { asm_pass0(polish); }); };
@@ -1597,14 +1369,14 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
             const int *L = d_lists + tofs[q];
             if (!exact) {
                 if (q == 0)
-                    vm_trial_kernel<vm_f8><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
+                    vm_launch_trial(false, nl, s, d_items, L, d_cand, d_rows, d_code, d_run_t, d_cp32, d_cp64, d_ploss);
                 else if (q == 1)
                     ft_trial_kernel<float, false><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
                 else
                     ft_trial_kernel<float, true><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
             } else {
                 if (q == 0)
-                    vm_trial_kernel<vm_d4><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
+                    vm_launch_trial(true, nl, s, d_items, L, d_cand, d_rows, d_code, d_run_t, d_cp32, d_cp64, d_ploss);
                 else if (q == 1)
                     ft_trial_kernel<double, false><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
                 else

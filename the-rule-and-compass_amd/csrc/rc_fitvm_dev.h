@@ -21,6 +21,7 @@
 #include "rc_fitvm.h"
 
 typedef unsigned long long vm_u64;
+typedef vm_u64 vm_u64x2 __attribute__((ext_vector_type(2)));
 typedef vm_u64 vm_u64x8 __attribute__((ext_vector_type(8)));
 typedef vm_u64 vm_u64x16 __attribute__((ext_vector_type(16)));
 typedef float vm_f2 __attribute__((ext_vector_type(2)));
@@ -52,6 +53,7 @@ typedef float vm_f8 __attribute__((ext_vector_type(8)));
 typedef double vm_d4 __attribute__((ext_vector_type(4)));
 
 template <typename F> struct VmSlots;
+template <> struct VmSlots<vm_u64x2> { enum { N = 2 }; };
 template <> struct VmSlots<vm_u64x8> { enum { N = 8 }; };
 template <> struct VmSlots<vm_u64x16> { enum { N = 16 }; };
 template <> struct VmSlots<dv8> { enum { N = 8 }; };
@@ -345,81 +347,84 @@ __device__ __forceinline__ V vm_forward(const uint2 *__restrict__ fc, int nf,
 /* ---- reverse sweep ----------------------------------------------------- */
 
 /* a * p with a zero adjoint contributing exactly 0 (ft_backward skips such
-   nodes). */
-__device__ __forceinline__ vm_f2 vm_zmul(vm_f2 a, vm_f2 p)
+   nodes), on every row of a pack. */
+template <typename V> __device__ __forceinline__ V vm_zmul(V a, V p)
 {
-    vm_f2 r;
-    r.x = (a.x == 0.0f) ? 0.0f : a.x * p.x;
-    r.y = (a.y == 0.0f) ? 0.0f : a.y * p.y;
+    V r;
+#pragma unroll
+    for (int q = 0; q < VmScalar<V>::R; ++q) r[q] = (a[q] == 0.0f) ? 0.0f : a[q] * p[q];
     return r;
 }
-__device__ __forceinline__ vm_f2 vm_zmul(vm_f2 a, float c)
+template <typename V> __device__ __forceinline__ V vm_zmul(V a, float c)
 {
-    vm_f2 r;
-    r.x = (a.x == 0.0f) ? 0.0f : a.x * c;
-    r.y = (a.y == 0.0f) ? 0.0f : a.y * c;
+    V r;
+#pragma unroll
+    for (int q = 0; q < VmScalar<V>::R; ++q) r[q] = (a[q] == 0.0f) ? 0.0f : a[q] * c;
     return r;
 }
 
-/* Gradient column write of two rows (row r and r + 64 of the block) plus the
-   finiteness accumulator (0 while every gradient is finite, NaN after). */
-__device__ __forceinline__ void vm_gwrite(char *xl, uint32_t x, vm_f2 g, vm_f2 &fin)
+/* Gradient column write of the pack's rows (rows lane + 64 q of the block)
+   plus the finiteness accumulator (0 while every gradient is finite, NaN
+   after). */
+template <typename V> __device__ __forceinline__ void vm_gwrite(char *xl, uint32_t x, V g, V &fin)
 {
     float *p = (float *)(xl + (x >> VMR_GOFS_SHIFT));
-    p[0] = g.x;
-    p[64] = g.y;
-    fin.x += g.x * 0.0f;
-    fin.y += g.y * 0.0f;
+#pragma unroll
+    for (int q = 0; q < VmScalar<V>::R; ++q) {
+        p[64 * q] = g[q];
+        fin[q] += g[q] * 0.0f;
+    }
 }
 
-/* Runs the reverse stream seeded with `seed`; gradient columns go to the LDS
-   block through xl (this lane's byte address of row 0, column 0).  Returns
-   the finiteness accumulator. */
-__device__ __forceinline__ vm_f2 vm_reverse(const uint2 *__restrict__ rc, int nr, vm_f2 seed,
-                                            const float *__restrict__ cst, const vm_u64x8 &X,
-                                            const vm_u64x16 &U, const vm_u64x16 &B, char *xl)
+/* Runs the reverse stream seeded with `seed` on a pack of f32 rows.  The
+   inputs are in U slots 0..v-1 (the stream was compiled with them there);
+   gradient columns go to the LDS block through xl (this lane's byte address
+   of row 0, column 0).  Returns the finiteness accumulator. */
+template <typename V, typename FU, typename FB, typename FA>
+__device__ __forceinline__ V vm_reverse(const uint2 *__restrict__ rc, int nr, V seed,
+                                        const float *__restrict__ cst, const FU &U0, const FU &U1,
+                                        const FB &B0, const FB &B1, FA &A0, FA &A1, char *xl)
 {
-    typedef VmPack<vm_f2> P;
-    vm_u64x8 Ad = (vm_u64x8)(0), none;
-    vm_f2 a = seed, fin = {0.0f, 0.0f};
+    typedef VmPack<V> P;
+    V a = seed, fin = vm_bcast<V>(0.0f);
     const uint32_t end = 8u * (uint32_t)nr;
     for (uint32_t o = 0; o < end; o += 8u) {
         const uint2 w = *(const uint2 *)((const char *)rc + o);
         const uint32_t x = w.x, y = w.y;
         if (x & VMR_UNMC_N) {
-            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, U, U, VMR_Y_UT(y))), fin);
-            a = vm_zmul(vm_zmul(a, vm_const(cst, y)), P::get(B, B, B, B, VMR_Y_BT(y)));
+            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y))), fin);
+            a = vm_zmul<V>(vm_zmul<V>(a, vm_const(cst, y)), P::get(B0, B1, B1, B1, VMR_Y_BT(y)));
         } else if (x & VMR_VMC) {
-            vm_gwrite(xl, x, vm_zmul(a, P::get(X, X, X, X, VMR_Y_XV(y))), fin);
-            a = P::get(Ad, Ad, Ad, Ad, VMR_Y_AR(y));
+            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_XV(y))), fin);
+            a = P::get(A0, A1, A1, A1, VMR_Y_AR(y));
         } else if (x & VMR_PUSH) {
-            P::set(Ad, none, none, none, VMR_Y_AS(y), a);
+            P::set(A0, A1, A1, A1, VMR_Y_AS(y), a);
         } else if (x & VMR_MULC_N) {
-            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, U, U, VMR_Y_UT(y))), fin);
-            a = vm_zmul(a, vm_const(cst, y));
+            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y))), fin);
+            a = vm_zmul<V>(a, vm_const(cst, y));
         } else if (x & VMR_UNMC_P) {
-            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, U, U, VMR_Y_UT(y))), fin);
-            a = P::get(Ad, Ad, Ad, Ad, VMR_Y_AR(y));
+            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y))), fin);
+            a = P::get(A0, A1, A1, A1, VMR_Y_AR(y));
         } else if (x & VMR_UN) {
-            a = vm_zmul(a, P::get(U, U, U, U, VMR_Y_UT(y)));
+            a = vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y)));
         } else if (x & VMR_MULC_P) {
-            vm_gwrite(xl, x, vm_zmul(a, P::get(U, U, U, U, VMR_Y_UT(y))), fin);
-            a = P::get(Ad, Ad, Ad, Ad, VMR_Y_AR(y));
+            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y))), fin);
+            a = P::get(A0, A1, A1, A1, VMR_Y_AR(y));
         } else if (x & VMR_MULV) {
-            a = vm_zmul(a, P::get(X, X, X, X, VMR_Y_XV(y)));
+            a = vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_XV(y)));
         } else if (x & VMR_ADDC_N) {
-            vm_gwrite(xl, x, a, fin);
+            vm_gwrite<V>(xl, x, a, fin);
         } else if (x & VMR_ADDC_P) {
-            vm_gwrite(xl, x, a, fin);
-            a = P::get(Ad, Ad, Ad, Ad, VMR_Y_AR(y));
+            vm_gwrite<V>(xl, x, a, fin);
+            a = P::get(A0, A1, A1, A1, VMR_Y_AR(y));
         } else {
             /* R_GEN: two word pairs. */
             o += 8u;
             const uint32_t x2 = *(const uint32_t *)((const char *)rc + o);
             const unsigned op = VMR_Y_OP(y);
-            vm_f2 gl, gr;
+            V gl, gr;
             if (x & VMR_UNARY) {
-                gl = vm_zmul(a, P::get(U, U, U, U, VMR_Y_UT(y)));
+                gl = vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y)));
                 gr = gl;
             } else if (op == OP_ADD) {
                 gl = a;
@@ -428,20 +433,20 @@ __device__ __forceinline__ vm_f2 vm_reverse(const uint2 *__restrict__ rc, int nr
                 gl = a;
                 gr = -a;
             } else {
-                gl = vm_zmul(a, P::get(U, U, U, U, VMR_Y_UT(y)));
-                gr = vm_zmul(a, P::get(B, B, B, B, VMR_Y_BT(y)));
+                gl = vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y)));
+                gr = vm_zmul<V>(a, P::get(B0, B1, B1, B1, VMR_Y_BT(y)));
             }
-            if (x & VMR_LCONST) vm_gwrite(xl, x, gl, fin);
-            if (x & VMR_RCONST) vm_gwrite(xl, x2, gr, fin);
+            if (x & VMR_LCONST) vm_gwrite<V>(xl, x, gl, fin);
+            if (x & VMR_RCONST) vm_gwrite<V>(xl, x2, gr, fin);
             if ((x & VMR_LNODE) && (x & VMR_RNODE)) {
-                P::set(Ad, none, none, none, VMR_Y_AS(y), gl);
+                P::set(A0, A1, A1, A1, VMR_Y_AS(y), gl);
                 a = gr;
             } else if (x & VMR_LNODE) {
                 a = gl;
             } else if (x & VMR_RNODE) {
                 a = gr;
             } else {
-                a = P::get(Ad, Ad, Ad, Ad, VMR_Y_AR(y));
+                a = P::get(A0, A1, A1, A1, VMR_Y_AR(y));
             }
         }
     }

@@ -1,0 +1,31 @@
+/*
+ * Launchers of the fit VM kernels (rc_fitvm_kern.hip) for the fit driver.
+ */
+#ifndef RC_FITVM_KERN_H
+#define RC_FITVM_KERN_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rc_fit_dev.h"
+
+/* LDS column stride of the VM assembly block (floats).  The reverse streams
+   carry gradient-column byte offsets built with this stride, and both row
+   shapes run the same streams (the f64 polish runs every VM candidate two
+   rows per lane), so both use it: 256 rows + 2 keeps the MFMA operand reads
+   of a half-wave on distinct banks. */
+#define VM_XS 258
+
+/* Assembly over the items of one class list: kc 16 / 32 Gram columns,
+   r = 2 or 4 rows per lane, polish = the f64-residual polish (2 rows). */
+void vm_launch_asm(int kc, int r, bool polish, int nl, hipStream_t s, const FtItem *items,
+                   const int *list, const FtCand *cands, const FtRows *rows, const uint32_t *code,
+                   const int *run, const float *cp32, const double *cp64, double *pgram,
+                   int tri_stride, double *ploss, int *pbad, int *pskip);
+
+/* Trial (f32, 8 rows per lane) or exact (f64, 4 rows per lane) loss. */
+void vm_launch_trial(bool exact, int nl, hipStream_t s, const FtItem *items, const int *list,
+                     const FtCand *cands, const FtRows *rows, const uint32_t *code, const int *run,
+                     const float *cp32, const double *cp64, double *ploss);
+
+#endif

@@ -1,0 +1,344 @@
+/*
+ * The fit VM kernels: assembly (forward + adjoint + FP32-MFMA Gram) and the
+ * value passes (trial and exact losses), compiled in their own translation
+ * unit.  They are single-wave interpreters dispatching on wave-uniform
+ * instruction words, so this file is built with uniform regions left
+ * unstructured (a dispatch test is one s_bitcmp and one branch); every
+ * other kernel of the library keeps the default structurization.
+ *
+ * Reference behaviour: ft_forward / ft_backward (eval.cu:1173-1317),
+ * ft_assemble (1693-1804) and ft_loss (1809-1855); the instruction streams
+ * come from the fit VM compiler (rc_fitvm.cpp).
+ */
+#include "rc_fit_dev.h"
+#include "rc_fitvm_dev.h"
+#include "rc_fitvm_kern.h"
+
+#include <math.h>
+
+namespace {
+
+/* Per-row residual bookkeeping shared by the assembly kernels. */
+struct RowPrep {
+    double r64, s64, wb;
+    bool live, ok;
+};
+
+__device__ __forceinline__ RowPrep vm_row_prep(bool live, double wb, float pred, double p64,
+                                               double y64, int metric, bool polish)
+{
+    RowPrep q;
+    q.live = live;
+    q.wb = wb;
+    q.r64 = p64 - y64;
+    q.ok = live && isfinite(p64) && (polish || fabsf(pred) <= 1e30f) && isfinite(q.r64);
+    q.s64 = sqrt(rc_row_weight(wb, q.r64, metric));
+    return q;
+}
+
+/* Row-pack shapes of the assembly: R rows per lane, a block of 64 R rows
+   and the register files. */
+template <int R> struct VmAsmShape;
+template <> struct VmAsmShape<2> {
+    typedef vm_f2 V;
+    typedef vm_u64x16 FU;
+    typedef vm_u64x16 FB;
+    typedef vm_u64x8 FA;
+};
+template <> struct VmAsmShape<4> {
+    typedef vm_f4 V;
+    typedef vm_u64x16 FU;
+    typedef vm_u64x8 FB;
+    typedef vm_u64x2 FA;
+};
+
+/* Assembly on the fit VM: the same per-row contract as ft_asm_kernel (the
+   augmented row [s*J | s*r_hi | s*r_lo] of every row into the Gram block,
+   the loss sum, bad-row and skipped-row flags), 64 R rows per wave step.
+   The streams hold the inputs in U slots 0..v-1.  For KC = 32 only the
+   K <= 22 live columns are stored; the MFMA reads of the padding columns
+   are clamped to the last stored one and land only in discarded entries. */
+template <int KC, bool POLISH, int R>
+__global__ __launch_bounds__(64) void vm_asm_kernel(
+    const FtItem *__restrict__ items, const int *__restrict__ list, int nlist,
+    const FtCand *__restrict__ cands, const FtRows *__restrict__ rowsd,
+    const uint32_t *__restrict__ code, const int *__restrict__ run, const float *__restrict__ cp32,
+    const double *__restrict__ cp64, double *__restrict__ pgram, int tri_stride,
+    double *__restrict__ ploss, int *__restrict__ pbad, int *__restrict__ pskip)
+{
+    typedef VmAsmShape<R> S;
+    typedef typename S::V V;
+    typedef typename S::FU FU;
+    typedef typename S::FB FB;
+    typedef typename S::FA FA;
+    typedef VmPack<V> P;
+    constexpr int XS = VM_XS;
+    constexpr int KCA = (KC == 32) ? FT_KMAX + 2 : KC; /* stored columns */
+    constexpr int BLOCK = 64 * R;
+    static_assert(!POLISH || R == 2, "the f64 polish runs two rows per lane");
+    const int li = blockIdx.x;
+    if (li >= nlist) return;
+    const int item = list[li];
+    const FtItem it = items[item];
+    const int cand = it.cand;
+    if (!run[cand]) return;
+    const FtCand d = cands[cand];
+    const FtRows Rw = rowsd[d.rowset];
+    const int lane = threadIdx.x;
+    const int k = d.k;
+
+    __shared__ float X[KCA * XS];
+    for (int i = lane; i < KCA * XS; i += 64) X[i] = 0.0f;
+    __syncthreads();
+
+    const float *cst = cp32 + (size_t)cand * FT_KMAX;
+    const double *cst64 = cp64 + (size_t)cand * FT_KMAX;
+    const uint2 *fc = (const uint2 *)(code + d.fwd_off);
+    const uint2 *rc = (const uint2 *)(code + d.rev_off);
+    char *xl = (char *)(X + lane);
+    floatx4 acc00 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
+    double g00[4] = {0, 0, 0, 0}, g10[4] = {0, 0, 0, 0}, g11[4] = {0, 0, 0, 0};
+    double lsum = 0.0;
+    int bad = 0, skip = 0;
+    FU U0 = (FU)(0), U1 = U0;
+    FB B0 = (FB)(0), B1 = B0;
+    FA A0 = (FA)(0), A1 = A0;
+    const int row_end = it.row0 + it.rows;
+    int step = 0;
+    for (int base = it.row0; base < row_end; base += BLOCK, ++step) {
+        int q[R];
+        bool in[R];
+        double wb[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = base + 64 * r + lane;
+            in[r] = row < row_end;
+            q[r] = in[r] ? row : it.row0;
+            wb[r] = Rw.w64 ? Rw.w64[q[r]] : 1.0;
+        }
+#pragma unroll
+        for (int v = 0; v < VM_MAX_VARS; ++v)
+            if (v < Rw.v) {
+                const float *col = Rw.cols + (size_t)v * (size_t)Rw.n;
+                V xv;
+#pragma unroll
+                for (int r = 0; r < R; ++r) xv[r] = col[q[r]];
+                P::set(U0, U1, U1, U1, v, xv);
+            }
+        const V pred = vm_forward<V, true>(fc, d.m, cst, U0, U1, U1, U1, U0, U1, U1, U1, B0, B1, B1,
+                                           B1);
+        double p64[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) p64[r] = (double)pred[r];
+        if (POLISH) {
+            dv16 Ua = (dv16)(0.0), Ub = Ua, Ba = Ua, Bb = Ua;
+#pragma unroll
+            for (int v = 0; v < VM_MAX_VARS; ++v)
+                if (v < Rw.v) {
+                    const double *col = Rw.cols64 + (size_t)v * (size_t)Rw.n;
+                    VmPack<vm_d2>::set(Ua, Ub, Ub, Ub, v, (vm_d2){col[q[0]], col[q[1 % R]]});
+                }
+            const vm_d2 pp = vm_forward<vm_d2, false>(fc, d.m, cst64, Ua, Ub, Ub, Ub, Ua, Ub, Ub,
+                                                      Ub, Ba, Bb, Bb, Bb);
+            p64[0] = pp.x;
+            p64[1 % R] = pp.y;
+        }
+        RowPrep a[R];
+        V seed;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            a[r] = vm_row_prep(in[r] && wb[r] > 0.0, wb[r], pred[r], p64[r], Rw.y64[q[r]], d.metric,
+                               POLISH);
+            seed[r] = a[r].ok ? (float)a[r].s64 : 0.0f;
+        }
+        const V fin = vm_reverse<V, FU, FB, FA>(rc, d.nrev, seed, cst, U0, U1, B0, B1, A0, A1, xl);
+        bool fix = false;
+        bool use[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const bool gf = fin[r] == 0.0f;
+            use[r] = a[r].ok && gf;
+            if (POLISH) {
+                if (a[r].live && !a[r].ok) bad = 1;
+                skip += (a[r].ok && !gf) ? 1 : 0;
+                if (a[r].ok) lsum += a[r].wb * rc_row_rho(a[r].r64, d.metric);
+            } else {
+                if (a[r].live && !use[r]) bad = 1;
+                if (use[r]) lsum += a[r].wb * rc_row_rho(a[r].r64, d.metric);
+            }
+            fix = fix || (a[r].ok && !gf);
+        }
+        if (__any(fix)) {
+            for (int j = 0; j < k; ++j)
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (!use[r]) X[j * XS + 64 * r + lane] = 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const double rs = a[r].s64 * a[r].r64;
+            const float rh = (float)rs;
+            X[k * XS + 64 * r + lane] = use[r] ? rh : 0.0f;
+            X[(k + 1) * XS + 64 * r + lane] = use[r] ? (float)(rs - (double)rh) : 0.0f;
+        }
+        __syncthreads();
+        const int col = lane & 15;
+        const int sub = lane >> 4;
+        const int col1 = (16 + col < KCA) ? 16 + col : KCA - 1;
+#pragma unroll 4
+        for (int qq = 0; qq < BLOCK / 4; ++qq) {
+            const float b0 = X[col * XS + 4 * qq + sub];
+            acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, b0, acc00, 0, 0, 0);
+            if (KC == 32) {
+                const float b1 = X[col1 * XS + 4 * qq + sub];
+                acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, b0, acc10, 0, 0, 0);
+                acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, b1, acc11, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+        /* promote to f64 every 256 rows */
+        if (R == 4 || (step & 1)) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                g00[r] += (double)acc00[r];
+                acc00[r] = 0.0f;
+                if (KC == 32) {
+                    g10[r] += (double)acc10[r];
+                    g11[r] += (double)acc11[r];
+                    acc10[r] = 0.0f;
+                    acc11[r] = 0.0f;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        g00[r] += (double)acc00[r];
+        if (KC == 32) {
+            g10[r] += (double)acc10[r];
+            g11[r] += (double)acc11[r];
+        }
+    }
+    const int K = k + 2;
+    double *pg = pgram + (size_t)item * (size_t)tri_stride;
+    const int j = lane & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = 4 * (lane >> 4) + r;
+        if (i >= j && i < K) pg[i * (i + 1) / 2 + j] = g00[r];
+        if (KC == 32) {
+            const int i1 = 16 + i;
+            if (i1 < K) pg[i1 * (i1 + 1) / 2 + j] = g10[r];
+            if (i >= j && i1 < K) pg[i1 * (i1 + 1) / 2 + 16 + j] = g11[r];
+        }
+    }
+    const double ls = rc_wave_sum_f64(lsum);
+    int sk = skip;
+    for (int off = 32; off > 0; off >>= 1) sk += __shfl_xor(sk, off, 64);
+    const int anybad = __any(bad) ? 1 : 0;
+    if (lane == 0) {
+        ploss[item] = ls;
+        pbad[item] = anybad;
+        pskip[item] = sk;
+    }
+}
+
+/* Trial / exact loss on the fit VM: the value-pass stream (inputs in U
+   slots 0..v-1, no tape), 8 f32 rows or 4 f64 rows per lane, with an 8-slot
+   U file of four 64-bit-element vectors. */
+template <typename V>
+__global__ __launch_bounds__(64) void vm_trial_kernel(
+    const FtItem *__restrict__ items, const int *__restrict__ list, int nlist,
+    const FtCand *__restrict__ cands, const FtRows *__restrict__ rowsd,
+    const uint32_t *__restrict__ code, const int *__restrict__ run,
+    const typename VmScalar<V>::T *__restrict__ cp, double *__restrict__ ploss)
+{
+    typedef typename VmScalar<V>::T T;
+    typedef VmPack<V> P;
+    typedef typename P::F8 F;
+    constexpr int RR = VmScalar<V>::R;
+    const int li = blockIdx.x;
+    if (li >= nlist) return;
+    const int item = list[li];
+    const FtItem it = items[item];
+    const int cand = it.cand;
+    if (!run[cand]) return;
+    const FtCand d = cands[cand];
+    const FtRows R_ = rowsd[d.rowset];
+    const int lane = threadIdx.x;
+    const T *cst = cp + (size_t)cand * FT_KMAX;
+    const uint2 *fc = (const uint2 *)(code + d.t_off);
+    const T *cols;
+    if (sizeof(T) == 4) cols = (const T *)(const void *)R_.cols;
+    else cols = (const T *)(const void *)R_.cols64;
+    F U0 = (F)(0), U1 = U0, U2 = U0, U3 = U0, Bd = U0;
+    double lsum = 0.0;
+    const int row_end = it.row0 + it.rows;
+    for (int base = it.row0; base < row_end; base += 64 * RR) {
+        int xrow[RR];
+        bool live[RR];
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            const int row = base + r * 64 + lane;
+            live[r] = row < row_end;
+            xrow[r] = live[r] ? row : it.row0;
+        }
+#pragma unroll
+        for (int v = 0; v < VM_MAX_VARS; ++v)
+            if (v < R_.v) {
+                const T *col = cols + (size_t)v * (size_t)R_.n;
+                V xv;
+#pragma unroll
+                for (int r = 0; r < RR; ++r) xv[r] = col[xrow[r]];
+                P::set(U0, U1, U2, U3, v, xv);
+            }
+        const V pred = vm_forward<V, false>(fc, d.t_m, cst, U0, U1, U2, U3, U0, U1, U2, U3, Bd, Bd,
+                                            Bd, Bd);
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            const double wb = R_.w64 ? R_.w64[xrow[r]] : 1.0;
+            if (!live[r] || !(wb > 0.0)) continue;
+            const double rr = (double)pred[r] - R_.y64[xrow[r]];
+            bool hard = !isfinite(pred[r]) || !isfinite(rr);
+            if (sizeof(T) == 4) hard = hard || fabs((double)pred[r]) > 1e30;
+            lsum += hard ? 1e30 : wb * rc_row_rho(rr, d.metric);
+        }
+    }
+    const double ls = rc_wave_sum_f64(lsum);
+    if (lane == 0) ploss[item] = ls;
+}
+
+} /* namespace */
+
+void vm_launch_asm(int kc, int r, bool polish, int nl, hipStream_t s, const FtItem *items,
+                   const int *list, const FtCand *cands, const FtRows *rows, const uint32_t *code,
+                   const int *run, const float *cp32, const double *cp64, double *pgram,
+                   int tri_stride, double *ploss, int *pbad, int *pskip)
+{
+    if (nl <= 0) return;
+#define VM_ASM(KC, P, R)                                                                        \
+    vm_asm_kernel<KC, P, R><<<nl, 64, 0, s>>>(items, list, nl, cands, rows, code, run, cp32, cp64, \
+                                             pgram, tri_stride, ploss, pbad, pskip)
+    if (polish) {
+        if (kc == 16) VM_ASM(16, true, 2);
+        else VM_ASM(32, true, 2);
+    } else if (r == 4) {
+        if (kc == 16) VM_ASM(16, false, 4);
+        else VM_ASM(32, false, 4);
+    } else {
+        if (kc == 16) VM_ASM(16, false, 2);
+        else VM_ASM(32, false, 2);
+    }
+#undef VM_ASM
+}
+
+void vm_launch_trial(bool exact, int nl, hipStream_t s, const FtItem *items, const int *list,
+                     const FtCand *cands, const FtRows *rows, const uint32_t *code, const int *run,
+                     const float *cp32, const double *cp64, double *ploss)
+{
+    if (nl <= 0) return;
+    if (exact)
+        vm_trial_kernel<vm_d4><<<nl, 64, 0, s>>>(items, list, nl, cands, rows, code, run, cp64, ploss);
+    else
+        vm_trial_kernel<vm_f8><<<nl, 64, 0, s>>>(items, list, nl, cands, rows, code, run, cp32, ploss);
+}
