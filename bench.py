@@ -38,6 +38,23 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+TRAFFIC_SUMMARY = os.path.join("profiles", "vm_asm_c3_traffic.md")
+
+
+def committed_traffic():
+    """HBM bytes per assembly launch at this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy of
+    known size); None when no summary is present."""
+    import re
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_SUMMARY)) as fh:
+            text = fh.read()
+    except OSError:
+        return None
+    m = re.search(r"hbm_bytes_per_launch:\s*(\d+)", text)
+    return int(m.group(1)) if m else None
+
+
 def copy_probe():
     """A plain 256 MiB device-to-device copy, three times: under rocprofv3
     --pmc FETCH_SIZE / WRITE_SIZE it gives the counters' reading for a known
@@ -155,12 +172,13 @@ def main():
         },
         "roofline": {
             "bound": "mfma",
-            "kernel": "ft_asm_kernel (forward + adjoint + FP32-MFMA Gram)",
+            "kernel": "vm_asm_kernel (fit VM forward + adjoint + FP32-MFMA Gram)",
             "achieved": asm_tflops,
             "peak": FP32_MFMA_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": asm_tflops / FP32_MFMA_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": committed_traffic(),
+            "traffic_source": TRAFFIC_SUMMARY if committed_traffic() is not None else None,
             "asm_ms_per_fit": rep.asm_ms,
             "asm_launches": rep.asm_launches,
             "trial_ms_per_fit": rep.trial_ms,

@@ -571,6 +571,15 @@ __global__ __launch_bounds__(64) void ft_probe_kernel(const FtCand *__restrict__
 
 __device__ __forceinline__ double clamp_step(double t) { return fmin(1e6, fmax(-1e6, t)); }
 
+__device__ __forceinline__ void ctrl_lane0(int op, int cand, FtState *s, const FtCand &d,
+                                        const double *__restrict__ c0, int kmax,
+                                        int *__restrict__ run_asm, int *__restrict__ run_trial,
+                                        float *__restrict__ cp32, double *__restrict__ cp64,
+                                        const double *G, double *L, double sh_loss, int sh_bad,
+                                        int sh_skip, int ran_trial, bool want_gram,
+                                        double *__restrict__ outc, double *__restrict__ outsse,
+                                        int *__restrict__ outstatus);
+
 __global__ __launch_bounds__(64) void ft_ctrl_kernel(
     int op, int ncand, FtState *__restrict__ S, const FtCand *__restrict__ cands,
     const double *__restrict__ c0, int kmax, int *__restrict__ run_asm, int *__restrict__ run_trial,
@@ -582,12 +591,24 @@ __global__ __launch_bounds__(64) void ft_ctrl_kernel(
     const int cand = blockIdx.x;
     if (cand >= ncand) return;
     const int lane = threadIdx.x;
-    FtState *s = S + cand;
     const FtCand d = cands[cand];
     __shared__ double G[FT_TRI_MAX];
     __shared__ double L[FT_KMAX * FT_KMAX];
     __shared__ double sh_loss;
     __shared__ int sh_bad, sh_skip;
+    /* The candidate's state is staged in LDS for this launch: the serial
+       control below (lane 0) and its solve then read and write LDS instead
+       of global memory. */
+    __shared__ FtState sS;
+    constexpr int SW = (int)(sizeof(FtState) / sizeof(int));
+    static_assert(sizeof(FtState) % sizeof(int) == 0, "FtState is copied as ints");
+    {
+        const int *src = (const int *)(S + cand);
+        int *dst = (int *)&sS;
+        for (int i = lane; i < SW; i += 64) dst[i] = src[i];
+    }
+    __syncthreads();
+    FtState *s = &sS;
 
     const int ran_asm = run_asm[cand];
     const int ran_trial = run_trial[cand];
@@ -628,8 +649,29 @@ __global__ __launch_bounds__(64) void ft_ctrl_kernel(
         sh_skip = sk;
     }
     __syncthreads();
-    if (lane != 0) return;
+    if (lane == 0) ctrl_lane0(op, cand, s, d, c0, kmax, run_asm, run_trial, cp32, cp64, G, L,
+                              sh_loss, sh_bad, sh_skip, ran_trial, want_gram, outc, outsse,
+                              outstatus);
+    __syncthreads();
+    {
+        const int *src = (const int *)&sS;
+        int *dst = (int *)(S + cand);
+        for (int i = lane; i < SW; i += 64) dst[i] = src[i];
+    }
+}
 
+/* The serial part of a control step (one lane): LM / IRLS / polish state
+   transitions on the candidate's LDS-staged state. */
+__device__ __forceinline__ void ctrl_lane0(int op, int cand, FtState *s, const FtCand &d,
+                                        const double *__restrict__ c0, int kmax,
+                                        int *__restrict__ run_asm, int *__restrict__ run_trial,
+                                        float *__restrict__ cp32, double *__restrict__ cp64,
+                                        const double *G, double *L, double sh_loss, int sh_bad,
+                                        int sh_skip, int ran_trial, bool want_gram,
+                                        double *__restrict__ outc, double *__restrict__ outsse,
+                                        int *__restrict__ outstatus)
+{
+    const int k = d.k;
     double *cc = s->c;
     auto set_pass32 = [&](const double *v) {
         for (int j = 0; j < FT_KMAX; ++j) cp32[(size_t)cand * FT_KMAX + j] = j < k ? (float)v[j] : 0.0f;
@@ -1158,8 +1200,9 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         const int vmrc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, VM_XS * 4,
                                            VMC_INPUTS_IN_U, vf.data(),
                                            (int)vf.size(), vr.data(), (int)vr.size(), info);
-        /* 4 rows per lane: an 8-slot B file and one saved adjoint. */
-        const bool vm4 = vmrc == 0 && info[3] <= 8 && info[4] <= 1;
+        /* 4 rows per lane: an 8-slot B file and one saved adjoint (and, for
+           the f64 polish, the value-pass stream, checked below). */
+        bool vm4 = vmrc == 0 && info[3] <= 8 && info[4] <= 1;
         if (vmrc == 0) {
             if (code.size() & 1) code.push_back(0);
             d.vm = 1;
@@ -1195,6 +1238,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
             code.insert(code.end(), progs[i].fwd.begin(), progs[i].fwd.end());
             d.t_m = (int)progs[i].fwd.size();
             tcls = progs[i].max_stack > SMALL_S ? 2 : 1;
+            vm4 = false;
         }
         const int n = c->sets[d.rowset].n;
         d.item0 = (int)items.size();
@@ -1341,9 +1385,8 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     };
     auto asm_pass0 = [&](int polish) {
         if (polish) {
-            /* the f64 polish residual runs two rows per lane for every VM class */
-            vm_asm(16, 2, true, 0);
-            vm_asm(32, 2, true, 1);
+            vm_asm(16, 4, true, 0);
+            vm_asm(32, 4, true, 1);
             vm_asm(16, 2, true, 2);
             vm_asm(32, 2, true, 3);
             asm_launch(ft_asm_kernel<16, false, true>, 4);

@@ -17,7 +17,8 @@
 #define VM_XS 258
 
 /* Assembly over the items of one class list: kc 16 / 32 Gram columns,
-   r = 2 or 4 rows per lane, polish = the f64-residual polish (2 rows). */
+   r = 2 or 4 rows per lane, polish = the f64-residual polish (with r = 4 the
+   residual runs on the candidate's value-pass stream). */
 void vm_launch_asm(int kc, int r, bool polish, int nl, hipStream_t s, const FtItem *items,
                    const int *list, const FtCand *cands, const FtRows *rows, const uint32_t *code,
                    const int *run, const float *cp32, const double *cp64, double *pgram,

@@ -75,7 +75,6 @@ __global__ __launch_bounds__(64) void vm_asm_kernel(
     constexpr int XS = VM_XS;
     constexpr int KCA = (KC == 32) ? FT_KMAX + 2 : KC; /* stored columns */
     constexpr int BLOCK = 64 * R;
-    static_assert(!POLISH || R == 2, "the f64 polish runs two rows per lane");
     const int li = blockIdx.x;
     if (li >= nlist) return;
     const int item = list[li];
@@ -130,7 +129,8 @@ __global__ __launch_bounds__(64) void vm_asm_kernel(
         double p64[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) p64[r] = (double)pred[r];
-        if (POLISH) {
+        if (POLISH && R == 2) {
+            /* f64 residual on the tape stream (16-slot file, 2 rows) */
             dv16 Ua = (dv16)(0.0), Ub = Ua, Ba = Ua, Bb = Ua;
 #pragma unroll
             for (int v = 0; v < VM_MAX_VARS; ++v)
@@ -142,6 +142,21 @@ __global__ __launch_bounds__(64) void vm_asm_kernel(
                                                       Ub, Ba, Bb, Bb, Bb);
             p64[0] = pp.x;
             p64[1 % R] = pp.y;
+        } else if (POLISH) {
+            /* f64 residual on the value-pass stream (8-slot file, 4 rows) */
+            const uint2 *tc = (const uint2 *)(code + d.t_off);
+            dv8 W0 = (dv8)(0.0), W1 = W0, W2 = W0, W3 = W0, Wb = W0;
+#pragma unroll
+            for (int v = 0; v < VM_MAX_VARS; ++v)
+                if (v < Rw.v) {
+                    const double *col = Rw.cols64 + (size_t)v * (size_t)Rw.n;
+                    VmPack<vm_d4>::set(W0, W1, W2, W3, v,
+                                       (vm_d4){col[q[0]], col[q[1 % R]], col[q[2 % R]], col[q[3 % R]]});
+                }
+            const vm_d4 pp = vm_forward<vm_d4, false>(tc, d.t_m, cst64, W0, W1, W2, W3, W0, W1, W2,
+                                                      W3, Wb, Wb, Wb, Wb);
+#pragma unroll
+            for (int r = 0; r < R; ++r) p64[r] = pp[r % 4];
         }
         RowPrep a[R];
         V seed;
@@ -319,7 +334,10 @@ void vm_launch_asm(int kc, int r, bool polish, int nl, hipStream_t s, const FtIt
 #define VM_ASM(KC, P, R)                                                                        \
     vm_asm_kernel<KC, P, R><<<nl, 64, 0, s>>>(items, list, nl, cands, rows, code, run, cp32, cp64, \
                                              pgram, tri_stride, ploss, pbad, pskip)
-    if (polish) {
+    if (polish && r == 4) {
+        if (kc == 16) VM_ASM(16, true, 4);
+        else VM_ASM(32, true, 4);
+    } else if (polish) {
         if (kc == 16) VM_ASM(16, true, 2);
         else VM_ASM(32, true, 2);
     } else if (r == 4) {
