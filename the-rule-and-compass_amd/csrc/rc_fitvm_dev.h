@@ -270,6 +270,40 @@ template <typename V> __device__ __forceinline__ V vm_partial(uint32_t x, uint32
     return r;
 }
 
+/* Value and partial of a unary node at once.  With VM_SINCOS the sin / cos
+   pair comes from one sincos evaluation (a shared argument reduction); that
+   is only enabled where sincosf's outputs have been checked to be the same
+   bits as sinf's and cosf's. */
+#ifndef VM_SINCOS
+#define VM_SINCOS 0
+#endif
+__device__ __forceinline__ void vm_sincos1(float a, float *s, float *c) { sincosf(a, s, c); }
+__device__ __forceinline__ void vm_sincos1(double a, double *s, double *c) { sincos(a, s, c); }
+template <typename V> __device__ __forceinline__ void vm_unary_and_partial(uint32_t x, uint32_t y, V a,
+                                                                          V &f, V &d)
+{
+    if (VM_SINCOS && (x & (VMF_SIN | VMF_COS))) {
+        V sn, cs;
+#pragma unroll
+        for (int q = 0; q < VmScalar<V>::R; ++q) {
+            typename VmScalar<V>::T s1, c1;
+            vm_sincos1(a[q], &s1, &c1);
+            sn[q] = s1;
+            cs[q] = c1;
+        }
+        if (x & VMF_SIN) {
+            f = sn;
+            d = cs;
+        } else {
+            f = cs;
+            d = -sn;
+        }
+        return;
+    }
+    f = vm_unary<V>(x, y, a);
+    d = vm_partial<V>(x, y, a, f);
+}
+
 /* ---- forward sweep ----------------------------------------------------- */
 
 /* Runs the forward stream on one pack of rows.  With TAPE the partials the
@@ -292,8 +326,14 @@ __device__ __forceinline__ V vm_forward(const uint2 *__restrict__ fc, int nf,
         V nt, uv = tos;
         uint32_t us = VMF_Y_US(y);
         if (x & VMF_UNMC) {
-            const V f = vm_unary<V>(x, y, tos);
-            if (TAPE && (x & VMF_NEED)) P::set(B0, B1, B2, B3, VMF_Y_BS(y), vm_partial<V>(x, y, tos, f));
+            V f;
+            if (TAPE && (x & VMF_NEED)) {
+                V dv;
+                vm_unary_and_partial<V>(x, y, tos, f, dv);
+                P::set(B0, B1, B2, B3, VMF_Y_BS(y), dv);
+            } else {
+                f = vm_unary<V>(x, y, tos);
+            }
             uv = f;
             nt = f * vm_const(cst, y);
         } else if (x & VMF_VMC) {
@@ -305,8 +345,9 @@ __device__ __forceinline__ V vm_forward(const uint2 *__restrict__ fc, int nf,
         } else if (x & VMF_PUSHV) {
             nt = P::get(X0, X1, X2, X3, VMF_Y_XV(y));
         } else if (x & VMF_UN) {
-            const V f = vm_unary<V>(x, y, tos);
-            if (TAPE && (x & VMF_NEED)) uv = vm_partial<V>(x, y, tos, f);
+            V f;
+            if (TAPE && (x & VMF_NEED)) vm_unary_and_partial<V>(x, y, tos, f, uv);
+            else f = vm_unary<V>(x, y, tos);
             nt = f;
         } else if (x & VMF_MULV) {
             nt = tos * P::get(X0, X1, X2, X3, VMF_Y_XV(y));
