@@ -146,10 +146,17 @@ class _File:
         self.w[i] = True
 
 
-def run(fwd, rev, c, X, col_bytes, flags=0):
+def run(fwd, rev, c, X, col_bytes, flags=0, bslots=None, aslots=None):
     """Forward value and constant gradients of every row of X (rows x vars).
     flags: the compiler options the streams were built with (inputs in U,
-    an 8-slot U file)."""
+    an 8-slot U file).  bslots / aslots: the sizes of the B and A files the
+    kernel runs the streams with (the 4-row assembly uses 8 and 2); slot
+    numbers are masked to the file size, as the device masks them.
+
+    The simulator writes every tape word's B slot and every general reverse
+    word's A slot, NaN where the word keeps nothing (the compiler names the
+    sink there), so a word that names a live slot where it should name the
+    sink corrupts the result."""
     X = np.asarray(X, np.float64)
     rows = X.shape[0]
     c = np.asarray(c, np.float64)
@@ -160,8 +167,11 @@ def run(fwd, rev, c, X, col_bytes, flags=0):
     if in_u:
         for q in range(X.shape[1]):
             U.set(q, X[:, q])
-    B = _File("B", D["VM_B_SLOTS"], rows, D["VM_B_SINK"])
-    A = _File("A", D["VM_A_SLOTS"], rows, D["VM_A_SINK"])
+    nb = bslots or D["VM_B_SLOTS"]
+    na = aslots or D["VM_A_SLOTS"]
+    B = _File("B", nb, rows, D["VM_B_SINK"] & (nb - 1))
+    A = _File("A", na, rows, D["VM_A_SINK"] & (na - 1))
+    garbage = np.full(rows, np.nan)
     tos = np.zeros(rows)
 
     def cval(y):
@@ -178,13 +188,13 @@ def run(fwd, rev, c, X, col_bytes, flags=0):
     for x, y in fw:
         x, y = int(x), int(y)
         us, ur, bs, xv, op = (y >> 8) & 15, (y >> 12) & 15, (y >> 16) & 15, (y >> 20) & 7, (y >> 23) & 31
+        bs &= nb - 1
         if small:
             ur &= 7
         uv = tos
         if x & D["VMF_UNMC"]:
             f = _unary(op, tos)
-            if x & D["VMF_NEED"]:
-                B.set(bs, _partial(op, tos, f))
+            B.set(bs, _partial(op, tos, f) if x & D["VMF_NEED"] else garbage)
             uv = f
             nt = f * cval(y)
         elif x & D["VMF_VMC"]:
@@ -214,11 +224,13 @@ def run(fwd, rev, c, X, col_bytes, flags=0):
             else:
                 l, r = U.get(ur), tos
             nt = _binary(op, l, r)
+            d1 = garbage
             if x & (D["VMF_D0"] | D["VMF_D1"]):
-                d0, d1 = _bpartials(op, l, r)
+                d0, d1p = _bpartials(op, l, r)
                 uv = d0
                 if x & D["VMF_D1"]:
-                    B.set(bs, d1)
+                    d1 = d1p
+            B.set(bs, d1)
         else:
             raise AssertionError(f"forward word without a handler: {x:#x}")
         if small:
@@ -251,9 +263,12 @@ def run(fwd, rev, c, X, col_bytes, flags=0):
         x, y = int(rw[i][0]), int(rw[i][1])
         ut, bt, as_, ar, xv, op = ((y >> 8) & 15, (y >> 12) & 15, (y >> 16) & 7, (y >> 19) & 7,
                                    (y >> 22) & 7, (y >> 25) & 31)
+        bt &= nb - 1
+        as_ &= na - 1
+        ar &= na - 1
 
         def pop():
-            return A.get(ar) if ar != D["VM_A_SINK"] else np.full(rows, np.nan)
+            return A.get(ar) if ar != A.sink else np.full(rows, np.nan)
 
         if x & D["VMR_UNMC_N"]:
             gw(x, z(a, U.get(ut)))
@@ -302,8 +317,8 @@ def run(fwd, rev, c, X, col_bytes, flags=0):
             if x & D["VMR_RCONST"]:
                 gw(x2, gr)
             ln, rn = bool(x & D["VMR_LNODE"]), bool(x & D["VMR_RNODE"])
+            A.set(as_, gl if (ln and rn) else garbage)
             if ln and rn:
-                A.set(as_, gl)
                 a = gr
             elif ln:
                 a = gl

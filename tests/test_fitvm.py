@@ -74,6 +74,14 @@ def check_tree(tree, X, rel=1e-9, flags=0):
             if not np.isfinite(g[j]):
                 continue
             assert abs(gv - g[j]) <= rel * (1.0 + abs(g[j])) * 10, (str(tree), r, j, gv, g[j])
+    if info[3] <= 7 and info[4] <= 1:
+        # On an 8-slot B file and a 2-slot A file (the 4-row assembly's) the
+        # sinks are slots 7 and 1: the streams still compute the same.
+        p4, g4 = fitvm_sim.run(fw, rv, plan.c0, X, COL_BYTES, flags, bslots=8, aslots=2)
+        assert np.array_equal(p4, pred, equal_nan=True), str(tree)
+        assert sorted(g4) == sorted(grads)
+        for j in grads:
+            assert np.array_equal(g4[j], grads[j], equal_nan=True), (str(tree), j)
     return 0
 
 
@@ -195,6 +203,30 @@ def test_simulator_catches_a_planted_slot_error(X):
     wrong[add, 1] = (int(wrong[add, 1]) & ~(15 << 12)) | (14 << 12)
     with pytest.raises(fitvm_sim.SlotError):
         fitvm_sim.run(wrong.ravel(), rv, plan.c0, X, COL_BYTES, 0)
+
+
+def test_simulator_catches_a_live_slot_named_instead_of_the_sink(X):
+    # A tape word that keeps no partial names the B sink.  The simulator
+    # writes NaN to the slot such a word names: plant the live slot of the
+    # sin partial into the cos word that keeps nothing, and the sin term's
+    # inner-constant gradient must come out wrong.
+    t = node(ADD, node(MUL, node(SIN, node(MUL, const(0.7), var(0))), const(1.3)),
+             node(MUL, node(COS, var(1)), const(0.4)))
+    rc, plan, fw, rv, info = compile_tree(t, 3, INPUTS_IN_U)
+    assert rc == 0
+    fw = fw.copy().reshape(-1, 2)
+    D = fitvm_sim.D
+    unmc = [i for i in range(len(fw)) if fw[i, 0] & D["VMF_UNMC"]]
+    need = [i for i in unmc if fw[i, 0] & D["VMF_NEED"]]
+    keep_nothing = [i for i in unmc if not fw[i, 0] & D["VMF_NEED"]]
+    assert len(need) == 1 and len(keep_nothing) == 1
+    live = (int(fw[need[0], 1]) >> 16) & 15
+    assert ((int(fw[keep_nothing[0], 1]) >> 16) & 15) == D["VM_B_SINK"]
+    _, good = fitvm_sim.run(fw.ravel(), rv, plan.c0, X, COL_BYTES, INPUTS_IN_U)
+    wrong = fw.copy()
+    wrong[keep_nothing[0], 1] = (int(wrong[keep_nothing[0], 1]) & ~(15 << 16)) | (live << 16)
+    _, bad = fitvm_sim.run(wrong.ravel(), rv, plan.c0, X, COL_BYTES, INPUTS_IN_U)
+    assert any(not np.array_equal(bad[j], good[j], equal_nan=True) for j in good)
 
 
 @pytest.mark.parametrize("flags", [INPUTS_IN_U, INPUTS_IN_U | VALUE_ONLY | SMALL_U])

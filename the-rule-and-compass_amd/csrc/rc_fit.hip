@@ -571,14 +571,22 @@ __global__ __launch_bounds__(64) void ft_probe_kernel(const FtCand *__restrict__
 
 __device__ __forceinline__ double clamp_step(double t) { return fmin(1e6, fmax(-1e6, t)); }
 
-__device__ __forceinline__ void ctrl_lane0(int op, int cand, FtState *s, const FtCand &d,
-                                        const double *__restrict__ c0, int kmax,
-                                        int *__restrict__ run_asm, int *__restrict__ run_trial,
-                                        float *__restrict__ cp32, double *__restrict__ cp64,
-                                        const double *G, double *L, double sh_loss, int sh_bad,
-                                        int sh_skip, int ran_trial, bool want_gram,
-                                        double *__restrict__ outc, double *__restrict__ outsse,
-                                        int *__restrict__ outstatus);
+/* A solve the serial control step leaves to the whole wave: the LM step at
+   the cached normal equations or the polish's Gauss-Newton step. */
+enum { CT_SOLVE_NONE = 0, CT_SOLVE_LM = 1, CT_SOLVE_POLISH = 2 };
+
+__device__ __forceinline__ int ctrl_lane0(int op, int cand, FtState *s, const FtCand &d,
+                                       const double *__restrict__ c0, int kmax,
+                                       float *__restrict__ cp32, double *__restrict__ cp64,
+                                       const double *G, double sh_loss, int sh_bad, int sh_skip,
+                                       int ran_trial, bool want_gram, double &solve_lambda,
+                                       double *__restrict__ outc, double *__restrict__ outsse,
+                                       int *__restrict__ outstatus);
+__device__ __forceinline__ void ctrl_after_solve(int kind, int rc, int cand, FtState *s, int k,
+                                              const double *delta, float *__restrict__ cp32,
+                                              double *__restrict__ cp64);
+__device__ __forceinline__ void ctrl_flags(int op, int cand, const FtState *s,
+                                        int *__restrict__ run_asm, int *__restrict__ run_trial);
 
 __global__ __launch_bounds__(64) void ft_ctrl_kernel(
     int op, int ncand, FtState *__restrict__ S, const FtCand *__restrict__ cands,
@@ -648,10 +656,24 @@ __global__ __launch_bounds__(64) void ft_ctrl_kernel(
         sh_bad = b;
         sh_skip = sk;
     }
+    __shared__ int sh_solve;
+    __shared__ double sh_lambda;
+    __shared__ double sh_delta[FT_KMAX], sh_dsc[FT_KMAX], sh_rhs[FT_KMAX];
     __syncthreads();
-    if (lane == 0) ctrl_lane0(op, cand, s, d, c0, kmax, run_asm, run_trial, cp32, cp64, G, L,
-                              sh_loss, sh_bad, sh_skip, ran_trial, want_gram, outc, outsse,
-                              outstatus);
+    if (lane == 0) {
+        double lam = 0.0;
+        sh_solve = ctrl_lane0(op, cand, s, d, c0, kmax, cp32, cp64, G, sh_loss, sh_bad, sh_skip,
+                              ran_trial, want_gram, lam, outc, outsse, outstatus);
+        sh_lambda = lam;
+    }
+    __syncthreads();
+    const int solve = sh_solve;
+    if (solve != CT_SOLVE_NONE) {
+        const int rc = ft_solve_wave(s->jtj, s->jtr, sh_lambda, k, sh_delta, L, sh_dsc, sh_rhs,
+                                     lane);
+        if (lane == 0) ctrl_after_solve(solve, rc, cand, s, k, sh_delta, cp32, cp64);
+    }
+    if (lane == 0) ctrl_flags(op, cand, s, run_asm, run_trial);
     __syncthreads();
     {
         const int *src = (const int *)&sS;
@@ -660,34 +682,36 @@ __global__ __launch_bounds__(64) void ft_ctrl_kernel(
     }
 }
 
+/* Staging of a pass's constants: f32 for the assembly / trial passes, f64
+   for the exact and polish passes (zero beyond k). */
+__device__ __forceinline__ void ctrl_set_pass32(float *__restrict__ cp32, int cand, int k,
+                                             const double *v)
+{
+    for (int j = 0; j < FT_KMAX; ++j) cp32[(size_t)cand * FT_KMAX + j] = j < k ? (float)v[j] : 0.0f;
+}
+__device__ __forceinline__ void ctrl_set_pass64(double *__restrict__ cp64, int cand, int k,
+                                             const double *v)
+{
+    for (int j = 0; j < FT_KMAX; ++j) cp64[(size_t)cand * FT_KMAX + j] = j < k ? v[j] : 0.0;
+}
+
 /* The serial part of a control step (one lane): LM / IRLS / polish state
-   transitions on the candidate's LDS-staged state. */
-__device__ __forceinline__ void ctrl_lane0(int op, int cand, FtState *s, const FtCand &d,
-                                        const double *__restrict__ c0, int kmax,
-                                        int *__restrict__ run_asm, int *__restrict__ run_trial,
-                                        float *__restrict__ cp32, double *__restrict__ cp64,
-                                        const double *G, double *L, double sh_loss, int sh_bad,
-                                        int sh_skip, int ran_trial, bool want_gram,
-                                        double *__restrict__ outc, double *__restrict__ outsse,
-                                        int *__restrict__ outstatus)
+   transitions on the candidate's LDS-staged state.  A step that needs a
+   solve stops there and returns its kind (with its lambda); the wave then
+   solves and ctrl_after_solve finishes the step. */
+__device__ __forceinline__ int ctrl_lane0(int op, int cand, FtState *s, const FtCand &d,
+                                       const double *__restrict__ c0, int kmax,
+                                       float *__restrict__ cp32, double *__restrict__ cp64,
+                                       const double *G, double sh_loss, int sh_bad, int sh_skip,
+                                       int ran_trial, bool want_gram, double &solve_lambda,
+                                       double *__restrict__ outc, double *__restrict__ outsse,
+                                       int *__restrict__ outstatus)
 {
     const int k = d.k;
     double *cc = s->c;
-    auto set_pass32 = [&](const double *v) {
-        for (int j = 0; j < FT_KMAX; ++j) cp32[(size_t)cand * FT_KMAX + j] = j < k ? (float)v[j] : 0.0f;
-    };
-    auto set_pass64 = [&](const double *v) {
-        for (int j = 0; j < FT_KMAX; ++j) cp64[(size_t)cand * FT_KMAX + j] = j < k ? v[j] : 0.0;
-    };
-    /* Solve at the cached normal equations and stage the trial. */
-    auto solve_to_trial = [&](double lambda) -> int {
-        double delta[FT_KMAX];
-        const int rc = ft_solve(s->jtj, s->jtr, lambda, k, delta, L);
-        if (rc == 1) return 1;
-        if (rc == 4) s->srank = 1;
-        for (int j = 0; j < k; ++j) s->trial[j] = clamp_step(cc[j] + delta[j]);
-        return 0;
-    };
+    int solve = CT_SOLVE_NONE;
+    auto set_pass32 = [&](const double *v) { ctrl_set_pass32(cp32, cand, k, v); };
+    auto set_pass64 = [&](const double *v) { ctrl_set_pass64(cp64, cand, k, v); };
     auto take_gram = [&]() {
         for (int i = 0; i < k; ++i) {
             for (int j = 0; j <= i; ++j) s->jtj[i * (i + 1) / 2 + j] = G[i * (i + 1) / 2 + j];
@@ -698,14 +722,10 @@ __device__ __forceinline__ void ctrl_lane0(int op, int cand, FtState *s, const F
         /* After an iteration: start the next one or end the loop. */
         if (s->stage == ST_LM_DONE) return;
         if (s->assembled) {
-            if (solve_to_trial(s->lambda) != 0) {
-                s->status = 1;
-                s->stage = ST_LM_DONE;
-                return;
-            }
-            set_pass32(s->trial);
-            s->halvings = s->irls ? 2 : 0;
-            s->stage = ST_NEED_TRIAL;
+            /* Solve at the cached normal equations (the wave's solve; the
+               trial is staged in ctrl_after_solve). */
+            solve = CT_SOLVE_LM;
+            solve_lambda = s->lambda;
         } else {
             set_pass32(cc);
             s->stage = ST_NEED_ASM;
@@ -826,10 +846,8 @@ __device__ __forceinline__ void ctrl_lane0(int op, int cand, FtState *s, const F
             if (sh_bad) { s->status = 2; s->stage = ST_DONE; break; }
             s->skipped += sh_skip;
             take_gram();
-            if (solve_to_trial(0.0) != 0) { s->stage = ST_DONE; break; }
-            set_pass64(s->trial);
-            s->halvings = s->irls ? 1 : 0;
-            s->stage = ST_P_NEED_TRIAL;
+            solve = CT_SOLVE_POLISH;
+            solve_lambda = 0.0;
             break;
         }
         case CT_AFTER_TRIAL_P: {
@@ -878,7 +896,41 @@ __device__ __forceinline__ void ctrl_lane0(int op, int cand, FtState *s, const F
         }
         default: break;
     }
-    /* Flags for the next pass. */
+    return solve;
+}
+
+/* The end of a control step that solved: stage the trial (clamped step from
+   the current constants) or record the failed solve. */
+__device__ __forceinline__ void ctrl_after_solve(int kind, int rc, int cand, FtState *s, int k,
+                                              const double *delta, float *__restrict__ cp32,
+                                              double *__restrict__ cp64)
+{
+    if (rc == 1) {
+        if (kind == CT_SOLVE_LM) {
+            s->status = 1;
+            s->stage = ST_LM_DONE;
+        } else {
+            s->stage = ST_DONE;
+        }
+        return;
+    }
+    if (rc == 4) s->srank = 1;
+    for (int j = 0; j < k; ++j) s->trial[j] = clamp_step(s->c[j] + delta[j]);
+    if (kind == CT_SOLVE_LM) {
+        ctrl_set_pass32(cp32, cand, k, s->trial);
+        s->halvings = s->irls ? 2 : 0;
+        s->stage = ST_NEED_TRIAL;
+    } else {
+        ctrl_set_pass64(cp64, cand, k, s->trial);
+        s->halvings = s->irls ? 1 : 0;
+        s->stage = ST_P_NEED_TRIAL;
+    }
+}
+
+/* Flags for the next pass, from the step's final stage. */
+__device__ __forceinline__ void ctrl_flags(int op, int cand, const FtState *s,
+                                        int *__restrict__ run_asm, int *__restrict__ run_trial)
+{
     const int stg = s->stage;
     switch (op) {
         case CT_INIT:
@@ -1184,7 +1236,8 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     std::vector<int> list[NA];
     std::vector<int> tlist[NT];
     int maxK = 2;
-    std::vector<uint32_t> vf(2 * 4 * FT_MAX_NODES), vr(2 * 8 * FT_MAX_NODES);
+    std::vector<uint32_t> vf(2 * 4 * FT_MAX_NODES), vr(2 * 8 * FT_MAX_NODES),
+        vt(2 * 4 * FT_MAX_NODES);
     for (int i = 0; i < nprog; ++i) {
         FtCand &d = cands[i];
         memset(&d, 0, sizeof(d));
@@ -1197,12 +1250,24 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         const int words = (int)(off[i + 1] - off[i]);
         int info[6] = {0, 0, 0, 0, 0, 0};
         const int nv = c->sets[d.rowset].v;
-        const int vmrc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, VM_XS * 4,
-                                           VMC_INPUTS_IN_U, vf.data(),
-                                           (int)vf.size(), vr.data(), (int)vr.size(), info);
-        /* 4 rows per lane: an 8-slot B file and one saved adjoint (and, for
-           the f64 polish, the value-pass stream, checked below). */
-        bool vm4 = vmrc == 0 && info[3] <= 8 && info[4] <= 1;
+        /* The value-pass stream: the fit VM's value-only form (inputs in U,
+           8-slot file) when it fits, else the general interpreter's (it has
+           no reverse stream, so no column stride). */
+        int tinfo[6] = {0, 0, 0, 0, 0, 0};
+        const int vorc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, VM_XS(4) * 4,
+                                           VMC_INPUTS_IN_U | VMC_VALUE_ONLY | VMC_SMALL_U,
+                                           vt.data(), (int)vt.size(), nullptr, 0, tinfo);
+        int vmrc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, VM_XS(4) * 4,
+                                     VMC_INPUTS_IN_U, vf.data(), (int)vf.size(), vr.data(),
+                                     (int)vr.size(), info);
+        /* 4 rows per lane: an 8-slot B file, one saved adjoint and, for the
+           f64 polish, the value-pass stream.  Otherwise 2 rows per lane,
+           whose block has the 2-row column stride. */
+        const bool vm4 = vmrc == 0 && info[3] <= 8 && info[4] <= 1 && vorc == 0;
+        if (vmrc == 0 && !vm4)
+            vmrc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, VM_XS(2) * 4,
+                                     VMC_INPUTS_IN_U, vf.data(), (int)vf.size(), vr.data(),
+                                     (int)vr.size(), info);
         if (vmrc == 0) {
             if (code.size() & 1) code.push_back(0);
             d.vm = 1;
@@ -1220,17 +1285,11 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
             d.m = (int)progs[i].fwd.size();
             d.nrev = (int)progs[i].rev.size();
         }
-        /* The value-pass stream: the fit VM's value-only form (inputs in U,
-           8-slot file) when it fits, else the general interpreter's. */
-        int tinfo[6] = {0, 0, 0, 0, 0, 0};
-        const int vorc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, VM_XS * 4,
-                                           VMC_INPUTS_IN_U | VMC_VALUE_ONLY | VMC_SMALL_U,
-                                           vf.data(), (int)vf.size(), nullptr, 0, tinfo);
         int tcls;
         if (vorc == 0) {
             if (code.size() & 1) code.push_back(0);
             d.t_off = (uint32_t)code.size();
-            code.insert(code.end(), vf.begin(), vf.begin() + 2 * tinfo[0]);
+            code.insert(code.end(), vt.begin(), vt.begin() + 2 * tinfo[0]);
             d.t_m = tinfo[0];
             tcls = 0;
         } else {
@@ -1238,7 +1297,6 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
             code.insert(code.end(), progs[i].fwd.begin(), progs[i].fwd.end());
             d.t_m = (int)progs[i].fwd.size();
             tcls = progs[i].max_stack > SMALL_S ? 2 : 1;
-            vm4 = false;
         }
         const int n = c->sets[d.rowset].n;
         d.item0 = (int)items.size();

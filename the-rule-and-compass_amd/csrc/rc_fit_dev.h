@@ -199,7 +199,8 @@ __device__ __forceinline__ T ft_unary_partial(unsigned op, T x, T out)
    delta~ = -D^-1/2 J^T r with the scale-free ln det gate on the undamped
    system (ft_solve, eval.cu:1874-1949).  jtj is the packed lower triangle
    (i*(i+1)/2 + j).  Returns 0 ok, 4 rank-deficient but solved, 1 no finite
-   step.  Serial: run by one lane. */
+   step.  Serial: run by one lane.  Every multiply-subtract is an explicit
+   fma, so the wave-parallel form below produces the same bits. */
 __device__ int ft_solve(const double *jtj, const double *jtr, double lambda, int p, double *delta,
                         double *L)
 {
@@ -216,11 +217,11 @@ __device__ int ft_solve(const double *jtj, const double *jtr, double lambda, int
         for (int j = 0; j <= i; ++j) {
             double s = jtj[i * (i + 1) / 2 + j] / (dscale[i] * dscale[j]);
             if (!isfinite(s)) return 1;
-            for (int q = 0; q < j; ++q) s -= L[i * FT_KMAX + q] * L[j * FT_KMAX + q];
+            for (int q = 0; q < j; ++q) s = fma(-L[i * FT_KMAX + q], L[j * FT_KMAX + q], s);
             if (i == j) {
                 if (!(s > 0.0) || !isfinite(s)) { ok = 0; break; }
                 L[i * FT_KMAX + i] = sqrt(s);
-                sum_ln_L += 0.5 * log(s);
+                sum_ln_L = fma(0.5, log(s), sum_ln_L);
             } else {
                 L[i * FT_KMAX + j] = s / L[j * FT_KMAX + j];
             }
@@ -235,8 +236,8 @@ __device__ int ft_solve(const double *jtj, const double *jtr, double lambda, int
     for (int i = 0; i < p; ++i) {
         for (int j = 0; j <= i; ++j) {
             double s = jtj[i * (i + 1) / 2 + j] / (dscale[i] * dscale[j]);
-            if (i == j) s = s * (1.0 + lambda) + 1e-12;
-            for (int q = 0; q < j; ++q) s -= L[i * FT_KMAX + q] * L[j * FT_KMAX + q];
+            if (i == j) s = fma(s, 1.0 + lambda, 1e-12);
+            for (int q = 0; q < j; ++q) s = fma(-L[i * FT_KMAX + q], L[j * FT_KMAX + q], s);
             if (i == j) {
                 if (!(s > 1e-18) || !isfinite(s)) return 1;
                 L[i * FT_KMAX + i] = sqrt(s);
@@ -249,15 +250,127 @@ __device__ int ft_solve(const double *jtj, const double *jtr, double lambda, int
     for (int i = 0; i < p; ++i) rhs[i] = -jtr[i] / dscale[i];
     for (int i = 0; i < p; ++i) {
         double s = rhs[i];
-        for (int q = 0; q < i; ++q) s -= L[i * FT_KMAX + q] * rhs[q];
+        for (int q = 0; q < i; ++q) s = fma(-L[i * FT_KMAX + q], rhs[q], s);
         rhs[i] = s / L[i * FT_KMAX + i];
     }
     for (int i = p - 1; i >= 0; --i) {
         double s = rhs[i];
-        for (int q = i + 1; q < p; ++q) s -= L[q * FT_KMAX + i] * delta[q];
+        for (int q = i + 1; q < p; ++q) s = fma(-L[q * FT_KMAX + i], delta[q], s);
         delta[i] = s / L[i * FT_KMAX + i];
     }
     for (int i = 0; i < p; ++i) delta[i] /= dscale[i];
+    return rank_deficient ? 4 : 0;
+}
+
+/* (i, j), j <= i, of packed lower-triangle index t = i (i + 1) / 2 + j. */
+__device__ __forceinline__ void ft_tri_index(int t, int &i, int &j)
+{
+    int r = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while ((r + 1) * (r + 2) / 2 <= t) ++r;
+    while (r * (r + 1) / 2 > t) --r;
+    i = r;
+    j = t - r * (r + 1) / 2;
+}
+
+/* One Cholesky factorization of the scaled lower triangle in S (row-major,
+   stride FT_KMAX), right-looking over the 64 lanes of a one-wave workgroup:
+   at step j the diagonal is checked and square-rooted, column j is divided
+   by it and the trailing triangle receives its q = j update.  Each entry thus
+   gets ft_solve's updates in increasing q, then its divide or root: the same
+   bits as the serial row-by-row order.  Returns the first row whose diagonal
+   fails (p when none); lane 0 accumulates 0.5 ln s of the accepted diagonals
+   in row order. */
+template <bool UNDAMPED>
+__device__ int ft_chol_wave(double *S, int p, int lane, double &sum_ln)
+{
+    for (int j = 0; j < p; ++j) {
+        const double d = S[j * FT_KMAX + j];
+        const bool good = UNDAMPED ? (d > 0.0 && isfinite(d)) : (d > 1e-18 && isfinite(d));
+        if (!good) return j; /* d is the same LDS word in every lane */
+        const double ljj = sqrt(d);
+        __syncthreads();
+        if (lane == 0) {
+            S[j * FT_KMAX + j] = ljj;
+            if (UNDAMPED) sum_ln = fma(0.5, log(d), sum_ln);
+        }
+        for (int i = j + 1 + lane; i < p; i += 64) S[i * FT_KMAX + j] = S[i * FT_KMAX + j] / ljj;
+        __syncthreads();
+        const int n = p - j - 1;
+        for (int t = lane; t < n * (n + 1) / 2; t += 64) {
+            int a, b;
+            ft_tri_index(t, a, b);
+            const int i = j + 1 + a, m = j + 1 + b;
+            S[i * FT_KMAX + m] = fma(-S[i * FT_KMAX + j], S[m * FT_KMAX + j], S[i * FT_KMAX + m]);
+        }
+        __syncthreads();
+    }
+    return p;
+}
+
+/* ft_solve run by the 64 lanes of a one-wave workgroup, with the same result
+   bits and status: both factorizations are wave-parallel (ft_chol_wave), the
+   serial solve's early exits are resolved in its order (a non-finite scaled
+   entry in a row the serial pass reaches returns 1 before that row's
+   diagonal is judged), and the two triangular solves run on lane 0 in
+   ft_solve's order.  jtj, jtr in any memory; S (FT_KMAX^2), dsc and rhs
+   (FT_KMAX) are LDS scratch; delta (LDS, FT_KMAX) is complete in every lane
+   on return.  The status is the same in every lane. */
+__device__ int ft_solve_wave(const double *jtj, const double *jtr, double lambda, int p,
+                             double *delta, double *S, double *dsc, double *rhs, int lane)
+{
+    int bad = 0;
+    if (lane < p) {
+        const double aii = jtj[lane * (lane + 1) / 2 + lane];
+        if (!(aii > 0.0) || !isfinite(aii)) bad = 1;
+        else dsc[lane] = sqrt(aii);
+    }
+    if (__any(bad)) return 1;
+    __syncthreads();
+    const int ntri = p * (p + 1) / 2;
+    unsigned nfrow = 0; /* rows holding a non-finite scaled entry */
+    for (int t = lane; t < ntri; t += 64) {
+        int i, j;
+        ft_tri_index(t, i, j);
+        const double s = jtj[t] / (dsc[i] * dsc[j]);
+        if (!isfinite(s)) nfrow |= 1u << i;
+        S[i * FT_KMAX + j] = s;
+    }
+    for (int off = 32; off > 0; off >>= 1) nfrow |= (unsigned)__shfl_xor((int)nfrow, off, 64);
+    __syncthreads();
+    double sum_ln = 0.0;
+    const int fail = ft_chol_wave<true>(S, p, lane, sum_ln);
+    const unsigned reached = fail >= p ? 0xffffffffu : (fail >= 31 ? 0xffffffffu : ((2u << fail) - 1u));
+    if (nfrow & reached) return 1;
+    int rank_deficient = 1;
+    if (fail >= p) {
+        const double ln_det = 2.0 * __shfl(sum_ln, 0, 64);
+        rank_deficient = (!isfinite(ln_det) || ln_det < FT_LN_DET_FLOOR) ? 1 : 0;
+    }
+    __syncthreads();
+    for (int t = lane; t < ntri; t += 64) {
+        int i, j;
+        ft_tri_index(t, i, j);
+        double s = jtj[t] / (dsc[i] * dsc[j]);
+        if (i == j) s = fma(s, 1.0 + lambda, 1e-12);
+        S[i * FT_KMAX + j] = s;
+    }
+    __syncthreads();
+    double unused = 0.0;
+    if (ft_chol_wave<false>(S, p, lane, unused) < p) return 1;
+    if (lane == 0) {
+        for (int i = 0; i < p; ++i) {
+            double s = -jtr[i] / dsc[i];
+            for (int q = 0; q < i; ++q) s = fma(-S[i * FT_KMAX + q], rhs[q], s);
+            rhs[i] = s / S[i * FT_KMAX + i];
+        }
+        for (int i = p - 1; i >= 0; --i) {
+            double s = rhs[i];
+            for (int q = i + 1; q < p; ++q) s = fma(-S[q * FT_KMAX + i], delta[q], s);
+            delta[i] = s / S[i * FT_KMAX + i];
+        }
+        for (int i = 0; i < p; ++i) delta[i] /= dsc[i];
+    }
+    __syncthreads();
     return rank_deficient ? 4 : 0;
 }
 

@@ -188,46 +188,70 @@ template <typename V> __device__ __forceinline__ V vm_bcast(typename VmScalar<V>
     return (V)(c);
 }
 
-/* ft_forward / value_of unary semantics on every row of a pack. */
-template <typename V> __device__ __noinline__ V vm_unary_call(unsigned op, V a)
+/* The out-of-line operator helpers receive the opcode in a VGPR (the calling
+   convention), though it is wave-uniform: reading it back with readfirstlane
+   turns their operator dispatch into scalar branches. */
+__device__ __forceinline__ unsigned vm_uniform(unsigned op)
 {
+    return (unsigned)__builtin_amdgcn_readfirstlane((int)op);
+}
+
+/* ft_forward / value_of unary semantics on every row of a pack. */
+template <typename V> __device__ __noinline__ V vm_unary_call(unsigned op_, V a)
+{
+    const unsigned op = vm_uniform(op_);
     V r;
 #pragma unroll
     for (int q = 0; q < VmScalar<V>::R; ++q) r[q] = ft_unary(op, a[q]);
     return r;
 }
 
-template <typename V> __device__ __noinline__ V vm_partial_call(unsigned op, V a, V f)
+template <typename V> __device__ __noinline__ V vm_partial_call(unsigned op_, V a, V f)
 {
+    const unsigned op = vm_uniform(op_);
     V r;
 #pragma unroll
     for (int q = 0; q < VmScalar<V>::R; ++q) r[q] = ft_unary_partial<typename VmScalar<V>::T>(op, a[q], f[q]);
     return r;
 }
 
-template <typename V> __device__ __noinline__ V vm_binary_call(unsigned op, V a, V b)
+template <typename V> __device__ __noinline__ V vm_binary_call(unsigned op_, V a, V b)
 {
+    const unsigned op = vm_uniform(op_);
     V r;
 #pragma unroll
     for (int q = 0; q < VmScalar<V>::R; ++q) r[q] = ft_binary(op, a[q], b[q]);
     return r;
 }
 
-/* Local partials of a binary node (ft_forward's tape values). */
-template <typename V> __device__ __noinline__ void vm_binary_partials(unsigned op, V a, V b, V &d0, V &d1)
+/* Local partials of a binary node (ft_forward's tape values); MUL's are its
+   operands and stay inline, the others are out of line. */
+template <typename V> __device__ __noinline__ void vm_binary_partials_call(unsigned op_, V a, V b,
+                                                                        V &d0, V &d1)
 {
     typedef typename VmScalar<V>::T T;
+    const unsigned op = vm_uniform(op_);
 #pragma unroll
     for (int q = 0; q < VmScalar<V>::R; ++q) {
         T p0 = (T)0, p1 = (T)0;
-        if (op == OP_MUL) { p0 = b[q]; p1 = a[q]; }
-        else if (op == OP_DIV) { p0 = (T)1 / b[q]; p1 = -a[q] / (b[q] * b[q]); }
+        if (op == OP_DIV) { p0 = (T)1 / b[q]; p1 = -a[q] / (b[q] * b[q]); }
         else if (op == OP_POW && a[q] > (T)0) {
             p0 = b[q] * pow(a[q], b[q] - (T)1);
             p1 = pow(a[q], b[q]) * log(a[q]);
         }
         d0[q] = p0;
         d1[q] = p1;
+    }
+}
+
+template <typename V> __device__ __forceinline__ void vm_binary_partials(unsigned op, V a, V b, V &d0,
+                                                                      V &d1)
+{
+    if (op == OP_MUL) {
+        d0 = b;
+        d1 = a;
+    } else {
+        vm_binary_partials_call<V>(op, a, b, d0, d1);
     }
 }
 
@@ -270,19 +294,17 @@ template <typename V> __device__ __forceinline__ V vm_partial(uint32_t x, uint32
     return r;
 }
 
-/* Value and partial of a unary node at once.  With VM_SINCOS the sin / cos
-   pair comes from one sincos evaluation (a shared argument reduction); that
-   is only enabled where sincosf's outputs have been checked to be the same
-   bits as sinf's and cosf's. */
-#ifndef VM_SINCOS
-#define VM_SINCOS 0
-#endif
+/* Value and partial of a unary node at once: the sin / cos pair comes from
+   one sincos evaluation (a shared argument reduction).  On gfx950 sincosf's
+   and sincos's two outputs are the same bits as sinf / cosf and sin / cos
+   (checked over 2^24 f32 arguments, |x| < 20, |x| < 2e4 and arbitrary bit
+   patterns, and their f64 widenings), so a fit does not change. */
 __device__ __forceinline__ void vm_sincos1(float a, float *s, float *c) { sincosf(a, s, c); }
 __device__ __forceinline__ void vm_sincos1(double a, double *s, double *c) { sincos(a, s, c); }
 template <typename V> __device__ __forceinline__ void vm_unary_and_partial(uint32_t x, uint32_t y, V a,
                                                                           V &f, V &d)
 {
-    if (VM_SINCOS && (x & (VMF_SIN | VMF_COS))) {
+    if (x & (VMF_SIN | VMF_COS)) {
         V sn, cs;
 #pragma unroll
         for (int q = 0; q < VmScalar<V>::R; ++q) {
@@ -304,6 +326,13 @@ template <typename V> __device__ __forceinline__ void vm_unary_and_partial(uint3
     d = vm_partial<V>(x, y, a, f);
 }
 
+/* One instruction word of a stream at byte offset o (a wave-uniform scalar
+   load). */
+__device__ __forceinline__ uint2 vm_word(const uint2 *__restrict__ s, uint32_t o)
+{
+    return *(const uint2 *)((const char *)s + o);
+}
+
 /* ---- forward sweep ----------------------------------------------------- */
 
 /* Runs the forward stream on one pack of rows.  With TAPE the partials the
@@ -320,8 +349,10 @@ __device__ __forceinline__ V vm_forward(const uint2 *__restrict__ fc, int nf,
     typedef VmPack<V> P;
     V tos = vm_bcast<V>(0);
     const uint32_t end = 8u * (uint32_t)nf;
+    /* The next word is loaded while the current one executes; its offset is
+       clamped to the last word of the stream. */
     for (uint32_t o = 0; o < end; o += 8u) {
-        const uint2 w = *(const uint2 *)((const char *)fc + o);
+        const uint2 w = vm_word(fc, o);
         const uint32_t x = w.x, y = w.y;
         V nt, uv = tos;
         uint32_t us = VMF_Y_US(y);
@@ -430,7 +461,7 @@ __device__ __forceinline__ V vm_reverse(const uint2 *__restrict__ rc, int nr, V 
     V a = seed, fin = vm_bcast<V>(0.0f);
     const uint32_t end = 8u * (uint32_t)nr;
     for (uint32_t o = 0; o < end; o += 8u) {
-        const uint2 w = *(const uint2 *)((const char *)rc + o);
+        const uint2 w = vm_word(rc, o);
         const uint32_t x = w.x, y = w.y;
         if (x & VMR_UNMC_N) {
             vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y))), fin);
@@ -459,9 +490,9 @@ __device__ __forceinline__ V vm_reverse(const uint2 *__restrict__ rc, int nr, V 
             vm_gwrite<V>(xl, x, a, fin);
             a = P::get(A0, A1, A1, A1, VMR_Y_AR(y));
         } else {
-            /* R_GEN: two word pairs. */
+            /* R_GEN: two word pairs; the second is the prefetched one. */
             o += 8u;
-            const uint32_t x2 = *(const uint32_t *)((const char *)rc + o);
+            const uint32_t x2 = vm_word(rc, o).x;
             const unsigned op = VMR_Y_OP(y);
             V gl, gr;
             if (x & VMR_UNARY) {

@@ -9,12 +9,12 @@
 
 #include "rc_fit_dev.h"
 
-/* LDS column stride of the VM assembly block (floats).  The reverse streams
-   carry gradient-column byte offsets built with this stride, and both row
-   shapes run the same streams (the f64 polish runs every VM candidate two
-   rows per lane), so both use it: 256 rows + 2 keeps the MFMA operand reads
-   of a half-wave on distinct banks. */
-#define VM_XS 258
+/* LDS column stride (floats) of the assembly block of R rows per lane: the
+   block's 64 R rows + 2, which keeps the MFMA operand reads of a half-wave on
+   distinct banks.  A candidate's reverse stream carries gradient-column byte
+   offsets built with the stride of the row shape its class runs (LM and
+   polish passes alike). */
+#define VM_XS(R) (64 * (R) + 2)
 
 /* Assembly over the items of one class list: kc 16 / 32 Gram columns,
    r = 2 or 4 rows per lane, polish = the f64-residual polish (with r = 4 the

@@ -72,7 +72,7 @@ __global__ __launch_bounds__(64) void vm_asm_kernel(
     typedef typename S::FB FB;
     typedef typename S::FA FA;
     typedef VmPack<V> P;
-    constexpr int XS = VM_XS;
+    constexpr int XS = VM_XS(R);
     constexpr int KCA = (KC == 32) ? FT_KMAX + 2 : KC; /* stored columns */
     constexpr int BLOCK = 64 * R;
     const int li = blockIdx.x;
