@@ -240,6 +240,9 @@ def run(fwd, rev, c, X, col_bytes, flags=0, bslots=None, aslots=None):
         U.v[us] = uv
         if us != usink:
             U.w[us] = True
+        if x & D["VMF_ACC"]:
+            # after the word's own U write: it may have spilled the left operand
+            nt = U.get(ur) + nt
         tos = nt
     pred = tos
 

@@ -185,13 +185,15 @@ def test_rejections():
 
 
 def test_simulator_catches_a_planted_slot_error(X):
-    # Plant a wrong read slot in the first ADD of a compiled program: the
-    # simulator must refuse it (read of an unwritten slot) or the values move.
+    # Plant a wrong read slot in the first add of a compiled program (a word
+    # with the fused ADD, ACC): the simulator must refuse it (read of an
+    # unwritten slot) or the values move.
     t = node(ADD, node(MUL, node(SIN, var(0)), const(1.1)), node(MUL, node(COS, var(1)), const(0.4)))
     rc, plan, fw, rv, info = compile_tree(t)
     assert rc == 0
     fw = fw.copy().reshape(-1, 2)
-    add = [i for i in range(len(fw)) if fw[i, 0] & fitvm_sim.D["VMF_ADD"]][0]
+    add = [i for i in range(len(fw))
+           if fw[i, 0] & (fitvm_sim.D["VMF_ADD"] | fitvm_sim.D["VMF_ACC"])][0]
     good, _ = fitvm_sim.run(fw.ravel(), rv, plan.c0, X, COL_BYTES, 0)
     ur = (int(fw[add, 1]) >> 12) & 15
     # another written slot: the value moves

@@ -71,6 +71,7 @@ struct Unit {
     int xv;               /* input read by the reverse */
     int gl, gr;           /* gradient columns (constant slots) or -1 */
     int nodes_next;       /* children that continue the sweep: 0, 1 or 2 */
+    bool ffused;          /* its forward word was folded into the previous one */
 };
 
 struct Compiler {
@@ -294,7 +295,18 @@ struct Compiler {
         if (dl) fx |= VMF_D0;
         if (dr) fx |= VMF_D1;
         if (N.op == O_ADD && !(fx & (VMF_RC | VMF_RV))) {
-            /* both operands computed: the dedicated add */
+            /* Both operands computed: the add folds into the word that
+               finished the right operand when that word leaves ur unused,
+               else the dedicated add. */
+            Unit &prev = units.back();
+            const uint32_t h = prev.fx;
+            const bool ur_free = !(h & (VMF_ADD | VMF_ACC)) &&
+                                 (!(h & VMF_BIN) || (h & (VMF_RC | VMF_RV)));
+            if (ur_free) {
+                prev.fx |= VMF_ACC;
+                prev.fy = (prev.fy & ~(0xfu << 12)) | (((uint32_t)ur & 0xfu) << 12);
+                u.ffused = true;
+            }
             fx = VMF_ADD;
         }
         u.fx = fx;
@@ -409,6 +421,7 @@ int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int 
        slots. */
     std::vector<uint32_t> rv;
     int adepth = 0, apeak = 0;
+    int popped = -1; /* the A slot the previous word ended by popping, else -1 */
     for (int i = C.vo ? -1 : (int)C.units.size() - 1; i >= 0; --i) {
         const Unit &u = C.units[(size_t)i];
         if (!u.rkind) continue;
@@ -419,10 +432,18 @@ int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int 
             if (adepth >= VM_A_SINK) return -3;
             as = adepth++;
             apeak = std::max(apeak, adepth);
+            if (u.rkind == VMR_PUSH && popped == as) {
+                /* The previous word popped the adjoint from this very slot:
+                   it still holds the same bits, so the push is dropped (the
+                   slot simply stays live). */
+                popped = -1;
+                continue;
+            }
         } else if (u.nodes_next == 0) {
             ar = adepth > 0 ? --adepth : VM_A_SINK;
             if (u.rkind == VMR_GEN) x |= VMR_POP;
         }
+        popped = (u.nodes_next == 0 && ar != VM_A_SINK) ? ar : -1;
         if (u.gl >= 0 && (u.gl + 1) * col_bytes >= (1 << (32 - VMR_GOFS_SHIFT))) return -3;
         if (u.gr >= 0 && (u.gr + 1) * col_bytes >= (1 << (32 - VMR_GOFS_SHIFT))) return -3;
         if (u.gl >= 0) x |= (uint32_t)(u.gl * col_bytes) << VMR_GOFS_SHIFT;
@@ -440,12 +461,16 @@ int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int 
             rv.push_back(0u);
         }
     }
-    const int nf = (int)C.units.size();
+    int nf = 0;
+    for (const Unit &u : C.units) nf += u.ffused ? 0 : 1;
     const int nr = (int)rv.size() / 2;
     if (!fwd || (!rev && nr > 0) || 2 * nf > cap_f || 2 * nr > cap_r) return -4;
-    for (int i = 0; i < nf; ++i) {
-        fwd[2 * i] = C.units[(size_t)i].fx;
-        fwd[2 * i + 1] = C.units[(size_t)i].fy;
+    int w = 0;
+    for (const Unit &u : C.units) {
+        if (u.ffused) continue;
+        fwd[2 * w] = u.fx;
+        fwd[2 * w + 1] = u.fy;
+        ++w;
     }
     if (!rv.empty()) memcpy(rev, rv.data(), rv.size() * 4);
     if (info) {

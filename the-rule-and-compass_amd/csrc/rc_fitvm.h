@@ -75,6 +75,11 @@
 #define VMF_RV (1u << 17)    /* BIN: right operand is the input X[xv]        */
 #define VMF_D0 (1u << 18)    /* BIN: store the left partial in U[us]         */
 #define VMF_D1 (1u << 19)    /* BIN: store the right partial in B[bs]        */
+/* After the handler and the word's U write: tos = U[ur] + tos, the ADD of a
+   spilled left operand and the right operand this word finished (on any word
+   whose handler does not read ur itself; when the word spills the left
+   operand itself, us == ur). */
+#define VMF_ACC (1u << 20)
 
 /* ---- reverse handlers (x bits 0..10) ----------------------------------- */
 /* R_UNMC_N  g = A*U[ut]; A = (A*c)*B[bt]                                    */

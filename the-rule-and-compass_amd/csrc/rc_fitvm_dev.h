@@ -349,14 +349,16 @@ __device__ __forceinline__ V vm_forward(const uint2 *__restrict__ fc, int nf,
     typedef VmPack<V> P;
     V tos = vm_bcast<V>(0);
     const uint32_t end = 8u * (uint32_t)nf;
-    /* The next word is loaded while the current one executes; its offset is
-       clamped to the last word of the stream. */
     for (uint32_t o = 0; o < end; o += 8u) {
         const uint2 w = vm_word(fc, o);
         const uint32_t x = w.x, y = w.y;
         V nt, uv = tos;
         uint32_t us = VMF_Y_US(y);
-        if (x & VMF_UNMC) {
+        /* Exactly one handler bit per word: the tests run in the order of
+           the handlers' frequency in fitted programs. */
+        if (x & VMF_VMC) {
+            nt = P::get(X0, X1, X2, X3, VMF_Y_XV(y)) * vm_const(cst, y);
+        } else if (x & VMF_UNMC) {
             V f;
             if (TAPE && (x & VMF_NEED)) {
                 V dv;
@@ -367,23 +369,21 @@ __device__ __forceinline__ V vm_forward(const uint2 *__restrict__ fc, int nf,
             }
             uv = f;
             nt = f * vm_const(cst, y);
-        } else if (x & VMF_VMC) {
-            nt = P::get(X0, X1, X2, X3, VMF_Y_XV(y)) * vm_const(cst, y);
-        } else if (x & VMF_ADD) {
-            nt = P::get(U0, U1, U2, U3, VMF_Y_UR(y)) + tos;
-        } else if (x & VMF_MULC) {
-            nt = tos * vm_const(cst, y);
+        } else if (x & VMF_MULV) {
+            nt = tos * P::get(X0, X1, X2, X3, VMF_Y_XV(y));
         } else if (x & VMF_PUSHV) {
             nt = P::get(X0, X1, X2, X3, VMF_Y_XV(y));
+        } else if (x & VMF_ADDC) {
+            nt = tos + vm_const(cst, y);
+        } else if (x & VMF_MULC) {
+            nt = tos * vm_const(cst, y);
         } else if (x & VMF_UN) {
             V f;
             if (TAPE && (x & VMF_NEED)) vm_unary_and_partial<V>(x, y, tos, f, uv);
             else f = vm_unary<V>(x, y, tos);
             nt = f;
-        } else if (x & VMF_MULV) {
-            nt = tos * P::get(X0, X1, X2, X3, VMF_Y_XV(y));
-        } else if (x & VMF_ADDC) {
-            nt = tos + vm_const(cst, y);
+        } else if (x & VMF_ADD) {
+            nt = P::get(U0, U1, U2, U3, VMF_Y_UR(y)) + tos;
         } else if (x & VMF_PUSHC) {
             nt = vm_bcast<V>(vm_const(cst, y));
         } else {
@@ -411,6 +411,9 @@ __device__ __forceinline__ V vm_forward(const uint2 *__restrict__ fc, int nf,
             }
         }
         P::set(U0, U1, U2, U3, us, uv);
+        /* After the U write: a word that spills the left operand itself
+           (us == ur) has just stored it. */
+        if (x & VMF_ACC) nt = P::get(U0, U1, U2, U3, VMF_Y_UR(y)) + nt;
         tos = nt;
     }
     return tos;
@@ -463,34 +466,35 @@ __device__ __forceinline__ V vm_reverse(const uint2 *__restrict__ rc, int nr, V 
     for (uint32_t o = 0; o < end; o += 8u) {
         const uint2 w = vm_word(rc, o);
         const uint32_t x = w.x, y = w.y;
-        if (x & VMR_UNMC_N) {
-            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y))), fin);
-            a = vm_zmul<V>(vm_zmul<V>(a, vm_const(cst, y)), P::get(B0, B1, B1, B1, VMR_Y_BT(y)));
+        /* One handler bit per word, tested in frequency order. */
+        if (x & VMR_PUSH) {
+            P::set(A0, A1, A1, A1, VMR_Y_AS(y), a);
         } else if (x & VMR_VMC) {
             vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_XV(y))), fin);
             a = P::get(A0, A1, A1, A1, VMR_Y_AR(y));
-        } else if (x & VMR_PUSH) {
-            P::set(A0, A1, A1, A1, VMR_Y_AS(y), a);
+        } else if (x & VMR_UNMC_N) {
+            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y))), fin);
+            a = vm_zmul<V>(vm_zmul<V>(a, vm_const(cst, y)), P::get(B0, B1, B1, B1, VMR_Y_BT(y)));
+        } else if (x & VMR_MULV) {
+            a = vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_XV(y)));
+        } else if (x & VMR_ADDC_N) {
+            vm_gwrite<V>(xl, x, a, fin);
         } else if (x & VMR_MULC_N) {
             vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y))), fin);
             a = vm_zmul<V>(a, vm_const(cst, y));
+        } else if (x & VMR_MULC_P) {
+            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y))), fin);
+            a = P::get(A0, A1, A1, A1, VMR_Y_AR(y));
         } else if (x & VMR_UNMC_P) {
             vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y))), fin);
             a = P::get(A0, A1, A1, A1, VMR_Y_AR(y));
         } else if (x & VMR_UN) {
             a = vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y)));
-        } else if (x & VMR_MULC_P) {
-            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y))), fin);
-            a = P::get(A0, A1, A1, A1, VMR_Y_AR(y));
-        } else if (x & VMR_MULV) {
-            a = vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_XV(y)));
-        } else if (x & VMR_ADDC_N) {
-            vm_gwrite<V>(xl, x, a, fin);
         } else if (x & VMR_ADDC_P) {
             vm_gwrite<V>(xl, x, a, fin);
             a = P::get(A0, A1, A1, A1, VMR_Y_AR(y));
         } else {
-            /* R_GEN: two word pairs; the second is the prefetched one. */
+            /* R_GEN: two word pairs. */
             o += 8u;
             const uint32_t x2 = vm_word(rc, o).x;
             const unsigned op = VMR_Y_OP(y);

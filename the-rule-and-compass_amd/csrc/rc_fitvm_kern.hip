@@ -200,15 +200,43 @@ __global__ __launch_bounds__(64) void vm_asm_kernel(
         const int col = lane & 15;
         const int sub = lane >> 4;
         const int col1 = (16 + col < KCA) ? 16 + col : KCA - 1;
-#pragma unroll 4
-        for (int qq = 0; qq < BLOCK / 4; ++qq) {
-            const float b0 = X[col * XS + 4 * qq + sub];
-            acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, b0, acc00, 0, 0, 0);
-            if (KC == 32) {
-                const float b1 = X[col1 * XS + 4 * qq + sub];
-                acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, b0, acc10, 0, 0, 0);
-                acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, b1, acc11, 0, 0, 0);
+        /* Double-buffered operands: the LDS reads of the next GQ MFMAs are
+           issued before the current GQ run (the scheduling fences keep them
+           there), so no MFMA waits on a fresh LDS read.  The MFMAs see the
+           same operands in the same order. */
+        constexpr int NQ = BLOCK / 4, GQ = 8;
+        static_assert(NQ % (2 * GQ) == 0, "operand batch pairs tile the block");
+        const float *x0 = X + col * XS + sub;
+        const float *x1 = X + col1 * XS + sub;
+        float p0[GQ], p1[GQ], q0[GQ], q1[GQ];
+        auto load = [&](float *d0, float *d1, int g) {
+#pragma unroll
+            for (int i = 0; i < GQ; ++i) {
+                d0[i] = x0[4 * (g + i)];
+                d1[i] = KC == 32 ? x1[4 * (g + i)] : 0.0f;
             }
+        };
+        auto gram = [&](const float *b0, const float *b1) {
+#pragma unroll
+            for (int i = 0; i < GQ; ++i) {
+                acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0[i], b0[i], acc00, 0, 0, 0);
+                if (KC == 32) {
+                    acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1[i], b0[i], acc10, 0, 0, 0);
+                    acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1[i], b1[i], acc11, 0, 0, 0);
+                }
+            }
+        };
+        load(p0, p1, 0);
+#pragma unroll
+        for (int g = 0; g < NQ; g += 2 * GQ) {
+            load(q0, q1, g + GQ);
+            __builtin_amdgcn_sched_barrier(0);
+            gram(p0, p1);
+            __builtin_amdgcn_sched_barrier(0);
+            if (g + 2 * GQ < NQ) load(p0, p1, g + 2 * GQ);
+            __builtin_amdgcn_sched_barrier(0);
+            gram(q0, q1);
+            __builtin_amdgcn_sched_barrier(0);
         }
         __syncthreads();
         /* promote to f64 every 256 rows */
