@@ -185,6 +185,44 @@ def test_rank_deficient_and_non_finite_fixtures_are_labelled():
     assert outs[0].status == 7
 
 
+def test_infinite_adjoint_times_zero_partial_flags_the_row():
+    # In sqrt(c0*x0) + c2*x1, a row with x0 = 0 has sqrt'(0) = inf: the
+    # adjoint reaching c0*x0 is infinite and its product with the zero
+    # partial x0 is NaN in ft_backward, so the row is flagged and the fit's
+    # start is unusable (status 7).  The mixed arm's zero-adjoint multiply
+    # (+0 for any zero operand) must flag it too: its status and constants
+    # match the plain FP64 arm's, whose products are guarded the reference's
+    # way; without such rows both arms fit to the same loss.
+    n = 2048
+    rng = np.random.default_rng(11)
+    x = np.stack([rng.uniform(0.2, 2.0, n), rng.uniform(-1.0, 1.0, n)], 1)
+    tree = node(ADD, node(SQRT, node(MUL, var(0), const(0.7))), node(MUL, var(1), const(-0.4)))
+    y = np.sqrt(0.9 * x[:, 0]) - 0.5 * x[:, 1] + rng.normal(0.0, 0.01, n)
+    plan = host.gpu_fit_plan(tree)
+    for zero_rows in (False, True):
+        xz = x.copy()
+        if zero_rows:
+            xz[::97, 0] = 0.0
+        res = {}
+        for mode in (host.FIT_MODE_MIXED, host.FIT_MODE_FP64):
+            ev = _evaluator(xz, y)
+            ev.set_fit_mode(mode)
+            outs, ms = ev.fit_batch([plan], 8, 0)
+            ev.close()
+            assert ms >= 0
+            res[mode] = outs[0]
+        m, f = res[host.FIT_MODE_MIXED], res[host.FIT_MODE_FP64]
+        _say(f"zero rows {zero_rows}: mixed status {m.status} sse {m.sse:.6e}; "
+             f"fp64 status {f.status} sse {f.sse:.6e}")
+        assert m.status == f.status
+        if zero_rows:
+            assert m.status == 7
+            assert m.c.tolist() == f.c.tolist()
+        else:
+            assert m.status in (0, 4)
+            assert m.sse == pytest.approx(f.sse, rel=1e-6)
+
+
 def test_iters_zero_keeps_the_start_and_reports_its_exact_loss():
     x, y, start = _linear_fixture(3, 300)
     ev = _evaluator(x, y)

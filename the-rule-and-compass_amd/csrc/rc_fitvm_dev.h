@@ -422,19 +422,35 @@ __device__ __forceinline__ V vm_forward(const uint2 *__restrict__ fc, int nf,
 /* ---- reverse sweep ----------------------------------------------------- */
 
 /* a * p with a zero adjoint contributing exactly 0 (ft_backward skips such
-   nodes), on every row of a pack. */
-template <typename V> __device__ __forceinline__ V vm_zmul(V a, V p)
+   nodes), on every row of a pack.  It is the legacy multiply (+0 whenever an
+   operand is +-0, Inf and NaN included) plus a * 0 folded into the row's
+   finiteness accumulator.  Against a guarded a * p this differs only
+
+   - for a non-finite adjoint: ft_backward's products of it are all
+     non-finite, so the row is flagged; here the a * 0 term flags it;
+   - in the sign of zero results (a != 0, p = +-0), which reach only Gram
+     sums starting from +0, zero adjoints and the accumulator;
+
+   so the assembled system and every row flag are the same. */
+__device__ float vm_fmul_legacy(float a, float b) __asm("llvm.amdgcn.fmul.legacy");
+template <typename V> __device__ __forceinline__ V vm_zmul(V a, V p, V &fin)
 {
     V r;
 #pragma unroll
-    for (int q = 0; q < VmScalar<V>::R; ++q) r[q] = (a[q] == 0.0f) ? 0.0f : a[q] * p[q];
+    for (int q = 0; q < VmScalar<V>::R; ++q) {
+        r[q] = vm_fmul_legacy(a[q], p[q]);
+        fin[q] += a[q] * 0.0f;
+    }
     return r;
 }
-template <typename V> __device__ __forceinline__ V vm_zmul(V a, float c)
+template <typename V> __device__ __forceinline__ V vm_zmul(V a, float c, V &fin)
 {
     V r;
 #pragma unroll
-    for (int q = 0; q < VmScalar<V>::R; ++q) r[q] = (a[q] == 0.0f) ? 0.0f : a[q] * c;
+    for (int q = 0; q < VmScalar<V>::R; ++q) {
+        r[q] = vm_fmul_legacy(a[q], c);
+        fin[q] += a[q] * 0.0f;
+    }
     return r;
 }
 
@@ -470,26 +486,26 @@ __device__ __forceinline__ V vm_reverse(const uint2 *__restrict__ rc, int nr, V 
         if (x & VMR_PUSH) {
             P::set(A0, A1, A1, A1, VMR_Y_AS(y), a);
         } else if (x & VMR_VMC) {
-            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_XV(y))), fin);
+            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_XV(y)), fin), fin);
             a = P::get(A0, A1, A1, A1, VMR_Y_AR(y));
         } else if (x & VMR_UNMC_N) {
-            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y))), fin);
-            a = vm_zmul<V>(vm_zmul<V>(a, vm_const(cst, y)), P::get(B0, B1, B1, B1, VMR_Y_BT(y)));
+            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y)), fin), fin);
+            a = vm_zmul<V>(vm_zmul<V>(a, vm_const(cst, y), fin), P::get(B0, B1, B1, B1, VMR_Y_BT(y)), fin);
         } else if (x & VMR_MULV) {
-            a = vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_XV(y)));
+            a = vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_XV(y)), fin);
         } else if (x & VMR_ADDC_N) {
             vm_gwrite<V>(xl, x, a, fin);
         } else if (x & VMR_MULC_N) {
-            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y))), fin);
-            a = vm_zmul<V>(a, vm_const(cst, y));
+            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y)), fin), fin);
+            a = vm_zmul<V>(a, vm_const(cst, y), fin);
         } else if (x & VMR_MULC_P) {
-            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y))), fin);
+            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y)), fin), fin);
             a = P::get(A0, A1, A1, A1, VMR_Y_AR(y));
         } else if (x & VMR_UNMC_P) {
-            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y))), fin);
+            vm_gwrite<V>(xl, x, vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y)), fin), fin);
             a = P::get(A0, A1, A1, A1, VMR_Y_AR(y));
         } else if (x & VMR_UN) {
-            a = vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y)));
+            a = vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y)), fin);
         } else if (x & VMR_ADDC_P) {
             vm_gwrite<V>(xl, x, a, fin);
             a = P::get(A0, A1, A1, A1, VMR_Y_AR(y));
@@ -500,7 +516,7 @@ __device__ __forceinline__ V vm_reverse(const uint2 *__restrict__ rc, int nr, V 
             const unsigned op = VMR_Y_OP(y);
             V gl, gr;
             if (x & VMR_UNARY) {
-                gl = vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y)));
+                gl = vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y)), fin);
                 gr = gl;
             } else if (op == OP_ADD) {
                 gl = a;
@@ -509,8 +525,8 @@ __device__ __forceinline__ V vm_reverse(const uint2 *__restrict__ rc, int nr, V 
                 gl = a;
                 gr = -a;
             } else {
-                gl = vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y)));
-                gr = vm_zmul<V>(a, P::get(B0, B1, B1, B1, VMR_Y_BT(y)));
+                gl = vm_zmul<V>(a, P::get(U0, U1, U1, U1, VMR_Y_UT(y)), fin);
+                gr = vm_zmul<V>(a, P::get(B0, B1, B1, B1, VMR_Y_BT(y)), fin);
             }
             if (x & VMR_LCONST) vm_gwrite<V>(xl, x, gl, fin);
             if (x & VMR_RCONST) vm_gwrite<V>(xl, x2, gr, fin);
