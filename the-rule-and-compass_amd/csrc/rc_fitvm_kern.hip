@@ -107,13 +107,16 @@ __global__ __launch_bounds__(64) void vm_asm_kernel(
     for (int base = it.row0; base < row_end; base += BLOCK, ++step) {
         int q[R];
         bool in[R];
-        double wb[R];
+        double wb[R], yv[R];
+        /* The step's weights and targets are loaded with its inputs, so
+           their latency overlaps the input loads. */
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int row = base + 64 * r + lane;
             in[r] = row < row_end;
             q[r] = in[r] ? row : it.row0;
             wb[r] = Rw.w64 ? Rw.w64[q[r]] : 1.0;
+            yv[r] = Rw.y64[q[r]];
         }
 #pragma unroll
         for (int v = 0; v < VM_MAX_VARS; ++v)
@@ -162,8 +165,7 @@ __global__ __launch_bounds__(64) void vm_asm_kernel(
         V seed;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            a[r] = vm_row_prep(in[r] && wb[r] > 0.0, wb[r], pred[r], p64[r], Rw.y64[q[r]], d.metric,
-                               POLISH);
+            a[r] = vm_row_prep(in[r] && wb[r] > 0.0, wb[r], pred[r], p64[r], yv[r], d.metric, POLISH);
             seed[r] = a[r].ok ? (float)a[r].s64 : 0.0f;
         }
         const V fin = vm_reverse<V, FU, FB, FA>(rc, d.nrev, seed, cst, U0, U1, B0, B1, A0, A1, xl);
@@ -337,14 +339,23 @@ __global__ __launch_bounds__(64) void vm_trial_kernel(
             }
         const V pred = vm_forward<V, false>(fc, d.t_m, cst, U0, U1, U2, U3, U0, U1, U2, U3, Bd, Bd,
                                             Bd, Bd);
+        /* All of the step's weights and targets are requested at once (one
+           wait), and a row that does not count adds +0.0 (the sum is never
+           -0), so the sum is the same as skipping it. */
+        double wv[RR], yv[RR];
 #pragma unroll
         for (int r = 0; r < RR; ++r) {
-            const double wb = R_.w64 ? R_.w64[xrow[r]] : 1.0;
-            if (!live[r] || !(wb > 0.0)) continue;
-            const double rr = (double)pred[r] - R_.y64[xrow[r]];
+            wv[r] = R_.w64 ? R_.w64[xrow[r]] : 1.0;
+            yv[r] = R_.y64[xrow[r]];
+        }
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            const double wb = wv[r];
+            const double rr = (double)pred[r] - yv[r];
             bool hard = !isfinite(pred[r]) || !isfinite(rr);
             if (sizeof(T) == 4) hard = hard || fabs((double)pred[r]) > 1e30;
-            lsum += hard ? 1e30 : wb * rc_row_rho(rr, d.metric);
+            const double term = hard ? 1e30 : wb * rc_row_rho(rr, d.metric);
+            lsum += (live[r] && wb > 0.0) ? term : 0.0;
         }
     }
     const double ls = rc_wave_sum_f64(lsum);
