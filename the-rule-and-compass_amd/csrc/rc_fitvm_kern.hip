@@ -24,6 +24,9 @@ struct RowPrep {
     bool live, ok;
 };
 
+/* UNIT: the squared metric with no row weights, where the weight is 1.0
+   and its square root exactly 1.0 (so the f64 root is not computed). */
+template <bool UNIT>
 __device__ __forceinline__ RowPrep vm_row_prep(bool live, double wb, float pred, double p64,
                                                double y64, int metric, bool polish)
 {
@@ -32,7 +35,7 @@ __device__ __forceinline__ RowPrep vm_row_prep(bool live, double wb, float pred,
     q.wb = wb;
     q.r64 = p64 - y64;
     q.ok = live && isfinite(p64) && (polish || fabsf(pred) <= 1e30f) && isfinite(q.r64);
-    q.s64 = sqrt(rc_row_weight(wb, q.r64, metric));
+    q.s64 = UNIT ? 1.0 : sqrt(rc_row_weight(wb, q.r64, metric));
     return q;
 }
 
@@ -103,6 +106,7 @@ __global__ __launch_bounds__(64) void vm_asm_kernel(
     FB B0 = (FB)(0), B1 = B0;
     FA A0 = (FA)(0), A1 = A0;
     const int row_end = it.row0 + it.rows;
+    const bool unit_w = Rw.w64 == nullptr && d.metric == RC_LOSS_SQUARED;
     int step = 0;
     for (int base = it.row0; base < row_end; base += BLOCK, ++step) {
         int q[R];
@@ -163,11 +167,19 @@ __global__ __launch_bounds__(64) void vm_asm_kernel(
         }
         RowPrep a[R];
         V seed;
+        if (unit_w) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            a[r] = vm_row_prep(in[r] && wb[r] > 0.0, wb[r], pred[r], p64[r], yv[r], d.metric, POLISH);
-            seed[r] = a[r].ok ? (float)a[r].s64 : 0.0f;
+            for (int r = 0; r < R; ++r)
+                a[r] = vm_row_prep<true>(in[r] && wb[r] > 0.0, wb[r], pred[r], p64[r], yv[r],
+                                         d.metric, POLISH);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                a[r] = vm_row_prep<false>(in[r] && wb[r] > 0.0, wb[r], pred[r], p64[r], yv[r],
+                                          d.metric, POLISH);
         }
+#pragma unroll
+        for (int r = 0; r < R; ++r) seed[r] = a[r].ok ? (float)a[r].s64 : 0.0f;
         const V fin = vm_reverse<V, FU, FB, FA>(rc, d.nrev, seed, cst, U0, U1, B0, B1, A0, A1, xl);
         bool fix = false;
         bool use[R];
