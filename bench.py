@@ -90,6 +90,8 @@ def main():
                     help="only the batched-evaluation part (one JSON object)")
     ap.add_argument("--eval-n", type=int, default=1048576)
     ap.add_argument("--eval-progs", type=int, default=4096)
+    ap.add_argument("--no-c4", action="store_true",
+                    help="skip the uneven-batch (autotuned vs fixed shapes) part")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device copy of 256 MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -252,7 +254,70 @@ def main():
     # f32 columns, inputs resident; the metric's row-evals/s.
     if not args.no_eval:
         result["eval"] = eval_part(args, host, workloads)
+    # Uneven batch (8 islands x 512, N log-uniform in [512, 4194304], mixed
+    # squared / L1): the built-in launch shapes against the autotuned ones.
+    if not args.no_c4:
+        result["c4_shapes"] = c4_part(host, workloads)
     print(json.dumps(result), flush=True)
+
+
+def c4_part(host, workloads):
+    import numpy as np
+    from rcsr_hip import autotune
+    ts = time.perf_counter()
+    wl = workloads.config4()
+    X, Y, sets = wl.x, wl.y, wl.rowsets
+    plans, isl, met = [], [], []
+    for t, i, m in zip(wl.trees, wl.island_of, wl.metrics):
+        p = host.gpu_fit_plan(t)
+        if p is None:
+            continue
+        plans.append(p)
+        isl.append(int(i))
+        met.append(int(m))
+    x0, y0 = X[sets[0]], Y[sets[0]]
+    ev = host.GpuEvaluator(np.ascontiguousarray(x0.T, np.float32).reshape(-1), len(x0), wl.v,
+                           y0.astype(np.float32))
+    assert ev.set_fit_data(np.ascontiguousarray(x0.T).reshape(-1), y0)
+    ids = {0: 0}
+    for s in range(1, len(sets)):
+        xs, ys = X[sets[s]], Y[sets[s]]
+        ids[s] = ev.add_rows(np.ascontiguousarray(xs.T, np.float32).reshape(-1), len(xs), wl.v,
+                             ys.astype(np.float32), np.ascontiguousarray(xs.T).reshape(-1), ys,
+                             None)
+    rowset = [ids[i] for i in isl]
+    rows = [len(sets[i]) for i in isl]
+    log(f"c4 workload ready in {time.perf_counter() - ts:.1f}s: {len(plans)} candidates, "
+        f"island rows {[len(r) for r in sets]}")
+
+    def fit_ms():
+        best = float("inf")
+        for _ in range(2):
+            outs, ms = ev.fit_batch_hetero(plans, rowset, met, wl.iters, kmax=20)
+            if ms < 0:
+                raise RuntimeError("rc_fit_batch_hetero failed")
+            best = min(best, ms)
+        return best
+    fixed = fit_ms()
+    ts = time.perf_counter()
+    res = autotune.tune_fit(ev, plans, wl.iters, rows=rows, rowset=rowset, metrics=met,
+                            sample=512, reps=1)
+    tune_s = time.perf_counter() - ts
+    tuned = fit_ms()
+    ev.close()
+    log(f"c4: built-in shapes {fixed:.1f} ms, autotuned {tuned:.1f} ms "
+        f"({res.to_json()['best']}, search {tune_s:.1f}s)")
+    return {
+        "workload": wl.name,
+        "candidates": len(plans),
+        "lm_iterations": wl.iters,
+        "fixed_ms": fixed,
+        "autotuned_ms": tuned,
+        "speedup": fixed / tuned,
+        "autotuned": res.to_json()["best"],
+        "search_seconds": tune_s,
+        "search_table_ms_on_512_candidate_sample": res.to_json()["table"],
+    }
 
 
 def tree_vars(t):

@@ -146,6 +146,24 @@ int rc_fit_kernel_info(int kmax, int *threads, int *smem_bytes, int *max_constan
    path (every row pass in f64).  Returns the previous mode or -1. */
 int rc_set_fit_mode(void *ctx, int mode);
 
+/* NEW, no counterpart in the reference: the launch shapes of this context,
+   chosen by the autotuner (rcsr_hip.autotune) or the caller; value 0 restores
+   the built-in choice.  A candidate's results depend on its own program and
+   rows and on these shapes, never on the rest of its batch.
+     RC_TUNE_FIT_TILE_ROWS      rows per fit work item: a multiple of 256 in
+                                [256, 4194304] (built-in 8192)
+     RC_TUNE_FIT_ROWS_PER_LANE  rows per lane of the fit VM assembly: 4 where
+                                the program fits the 4-row files (built-in) or 2
+     RC_TUNE_EVAL_TILE_ROWS     rows per evaluation tile: a multiple of 256 in
+                                [256, 4194304] (built-in 4096)
+   rc_set_tuning returns the previous value, or -1 for an unknown key or a
+   value out of range; rc_get_tuning returns the current value or -1. */
+#define RC_TUNE_FIT_TILE_ROWS 1
+#define RC_TUNE_FIT_ROWS_PER_LANE 2
+#define RC_TUNE_EVAL_TILE_ROWS 3
+int rc_set_tuning(void *ctx, int key, int value);
+int rc_get_tuning(void *ctx, int key);
+
 /* NEW: add a resident row set for heterogeneous batches; returns its id
    (the rc_create set is id 0) or -1.  cols/y are f32 column-major, cols64/y64
    their exact f64 twins, w optional base weights. */

@@ -361,4 +361,43 @@ int rc_device_count(void)
     return n;
 }
 
+/* Launch shapes (no counterpart in the reference; the autotuner sets them). */
+int rc_set_tuning(void *ctx, int key, int value)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c) return -1;
+    const bool tile_ok = value == 0 || (value >= 256 && value <= (1 << 22) && value % 256 == 0);
+    int *slot = nullptr;
+    switch (key) {
+        case RC_TUNE_FIT_TILE_ROWS:
+            if (!tile_ok) return -1;
+            slot = &c->tune_fit_tile;
+            break;
+        case RC_TUNE_FIT_ROWS_PER_LANE:
+            if (value != 0 && value != 2 && value != 4) return -1;
+            slot = &c->tune_fit_rows;
+            break;
+        case RC_TUNE_EVAL_TILE_ROWS:
+            if (!tile_ok) return -1;
+            slot = &c->tune_eval_tile;
+            break;
+        default: return -1;
+    }
+    const int prev = *slot;
+    *slot = value;
+    return prev;
+}
+
+int rc_get_tuning(void *ctx, int key)
+{
+    const RcCtx *c = (const RcCtx *)ctx;
+    if (!c) return -1;
+    switch (key) {
+        case RC_TUNE_FIT_TILE_ROWS: return c->tune_fit_tile;
+        case RC_TUNE_FIT_ROWS_PER_LANE: return c->tune_fit_rows;
+        case RC_TUNE_EVAL_TILE_ROWS: return c->tune_eval_tile;
+        default: return -1;
+    }
+}
+
 } /* extern "C" */

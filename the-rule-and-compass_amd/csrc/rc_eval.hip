@@ -421,10 +421,11 @@ namespace {
 /* Row tiles are a multiple of 256 rows so every class (R = 4, 2, 1) walks
    whole passes; tiles depend only on n, so a program's sums never depend on
    the batch it came in. */
-int tile_rows_for(int n)
+int tile_rows_for(int n, int tuned)
 {
-    if (n <= 4096) return ((n + 255) / 256) * 256 > 0 ? ((n + 255) / 256) * 256 : 256;
-    return 4096;
+    const int whole = ((n + 255) / 256) * 256 > 0 ? ((n + 255) / 256) * 256 : 256;
+    const int t = tuned > 0 ? tuned : 4096;
+    return whole < t ? whole : t;
 }
 
 template <int S, int R>
@@ -450,7 +451,7 @@ int launch_rows(RcCtx *c, int mode, int kind, int nprog, const uint32_t *code,
     RcRows *r = &c->sets[0];
     const int n = r->n;
     if (n <= 0) return -1;
-    const int tile_rows = tile_rows_for(n);
+    const int tile_rows = tile_rows_for(n, c->tune_eval_tile);
     const int tiles = (n + tile_rows - 1) / tile_rows;
     const size_t items = (size_t)nprog * (size_t)tiles;
     if (rc_grow(c, (void **)&c->d_part, &c->part_cap, items, sizeof(double)) != 0) return -1;

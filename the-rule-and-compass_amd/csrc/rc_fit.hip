@@ -42,7 +42,7 @@
 namespace {
 
 constexpr int XS = 66;          /* LDS column stride of the augmented rows */
-constexpr int TILE_ROWS = 8192; /* rows per work item (multiple of 256) */
+constexpr int TILE_ROWS = 8192; /* built-in rows per work item (a multiple of 256) */
 /* Interpreter capacities.  Every vector update is unconditional (the op's
    arithmetic runs on scalars), so a slot above the live area is written each
    step: the small class needs depth + 1 value slots, adjoint depth + 2 slots
@@ -1238,6 +1238,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     int maxK = 2;
     std::vector<uint32_t> vf(2 * 4 * FT_MAX_NODES), vr(2 * 8 * FT_MAX_NODES),
         vt(2 * 4 * FT_MAX_NODES);
+    const int tile_rows = c->tune_fit_tile > 0 ? c->tune_fit_tile : TILE_ROWS;
     for (int i = 0; i < nprog; ++i) {
         FtCand &d = cands[i];
         memset(&d, 0, sizeof(d));
@@ -1263,7 +1264,8 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         /* 4 rows per lane: an 8-slot B file, one saved adjoint and, for the
            f64 polish, the value-pass stream.  Otherwise 2 rows per lane,
            whose block has the 2-row column stride. */
-        const bool vm4 = vmrc == 0 && info[3] <= 8 && info[4] <= 1 && vorc == 0;
+        const bool vm4 = vmrc == 0 && info[3] <= 8 && info[4] <= 1 && vorc == 0 &&
+                         c->tune_fit_rows != 2;
         if (vmrc == 0 && !vm4)
             vmrc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, VM_XS(2) * 4,
                                      VMC_INPUTS_IN_U, vf.data(), (int)vf.size(), vr.data(),
@@ -1304,11 +1306,11 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         const int kc = d.k + 2 <= 16 ? 0 : 1;
         const int cls = d.vm ? (vm4 ? 0 : 2) + kc : (big ? 6 : 4) + kc;
         maxK = std::max(maxK, d.k + 2);
-        for (int r0 = 0; r0 < n; r0 += TILE_ROWS) {
+        for (int r0 = 0; r0 < n; r0 += tile_rows) {
             FtItem it;
             it.cand = i;
             it.row0 = r0;
-            it.rows = std::min(TILE_ROWS, n - r0);
+            it.rows = std::min(tile_rows, n - r0);
             it.pad = 0;
             list[cls].push_back((int)items.size());
             tlist[tcls].push_back((int)items.size());

@@ -41,6 +41,12 @@ struct RcFitState; /* rc_fit.hip */
 struct RcCtx {
     RcRows sets[RC_MAX_SETS];
     int nsets = 1;
+    /* Launch shapes set by rc_set_tuning (0 = the built-in choice): rows per
+       fit work item, rows per lane of the fit VM assembly, rows per
+       evaluation tile. */
+    int tune_fit_tile = 0;
+    int tune_fit_rows = 0;
+    int tune_eval_tile = 0;
     float *d_tsign = nullptr;
     double *d_tsign64 = nullptr;
     int has_tsign = 0;

@@ -34,6 +34,8 @@ GPU_FIT_MAX_NODES = 96       # fit.rs:854
 METRIC_SQUARED, METRIC_LOG_SQUASHED, METRIC_ABSOLUTE = 0, 1, 4
 
 FIT_MODE_MIXED, FIT_MODE_FP64 = 0, 1
+# rc_set_tuning keys (include/rcsr_hip.h).
+TUNE_FIT_TILE_ROWS, TUNE_FIT_ROWS_PER_LANE, TUNE_EVAL_TILE_ROWS = 1, 2, 3
 
 
 class CompileError(Exception):
@@ -216,6 +218,14 @@ class GpuEvaluator:
         return self._L.rc_add_rows(self.ctx, _L.ptr(cols, _L.f32p), int(n), int(v),
                                    _L.ptr(y, _L.f32p), _L.ptr(c64, _L.f64p), _L.ptr(yy, _L.f64p),
                                    _L.ptr(ww, _L.f64p))
+
+    def set_tuning(self, key: int, value: int) -> int:
+        """Set a launch shape (TUNE_*; 0 = the built-in choice); returns the
+        previous value, -1 when the key or value is refused."""
+        return int(self._L.rc_set_tuning(self.ctx, int(key), int(value)))
+
+    def get_tuning(self, key: int) -> int:
+        return int(self._L.rc_get_tuning(self.ctx, int(key)))
 
     def set_fit_mode(self, mode: int) -> int:
         return self._L.rc_set_fit_mode(self.ctx, int(mode))

@@ -68,6 +68,8 @@ SIGNATURES = {
                                  C.c_int, f64p, f64p, i32p]),
     "rc_fit_kernel_info": (C.c_int, [C.c_int, i32p, i32p, i32p]),
     "rc_set_fit_mode": (C.c_int, [vp, C.c_int]),
+    "rc_set_tuning": (C.c_int, [vp, C.c_int, C.c_int]),
+    "rc_get_tuning": (C.c_int, [vp, C.c_int]),
     "rc_add_rows": (C.c_int, [vp, f32p, C.c_int, C.c_int, f32p, f64p, f64p, f64p]),
     "rc_fit_batch_hetero": (C.c_float, [vp, u32p, u32p, i32p, C.c_int, C.c_int, f64p, i32p, i32p,
                                         C.c_int, f64p, f64p, i32p]),
