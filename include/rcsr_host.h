@@ -41,6 +41,19 @@ int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int 
                       unsigned int flags, unsigned int *fwd, int cap_f, unsigned int *rev,
                       int cap_r, int *info);
 
+/* No reference counterpart: the evaluator's program translation (what
+   rc_set_programs uploads), exported so it can be checked on the CPU.
+   prog / off / nprog: the rc_set_programs batch; nvars: resident columns.
+   code receives [per-program streams | hoist programs], meta receives
+   [stream offset per program | program list grouped by stack class | pairs
+   (hoist program offset, constant word to fill)]; info[0..6] = code words,
+   meta words, the four class starts and the batch's peak stack depth (filled
+   even when a buffer is too small).  Returns 0, -1 bad arguments, -2 a
+   buffer is too small. */
+int rch_eval_translate(const unsigned int *prog, const unsigned int *off, int nprog, int nvars,
+                       unsigned int *code, int cap_code, unsigned int *meta, int cap_meta,
+                       int *info);
+
 #ifdef __cplusplus
 }
 #endif

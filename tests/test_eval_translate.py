@@ -1,0 +1,308 @@
+"""CPU checks of the evaluator's program translation (rch_eval_translate, the
+streams rc_set_programs uploads).
+
+The translation rebuilds each program's tree, evaluates input-free subtrees
+once per batch (hoist programs whose results fill constant words) and fuses
++, -, *, / with a leaf operand into one word.  A small f32 simulator of the
+device interpreter runs the translated streams, hoist programs first, and
+must reproduce, bit for bit, a plain f32 stack machine run over the original
+postfix program with the reference's stack rules (rc_exec_row,
+eval.cu:404-573: underflow, overflow past 64 and a missing HALT fail every
+row; the value returned at HALT is the bottom stack entry).  Both machines
+share one set of f32 operator functions, so any difference is the
+translation's: operand order, fusion, hoisting, dropped dead entries.
+"""
+import numpy as np
+import pytest
+
+from rcsr_hip import host, lib, workloads
+from rcsr_hip.expr import (ADD, ATAN2, COS, DIV, EXP, HALT, IF, LOG, MAX, MIN, MOD, MUL, NEG,
+                           POW, SIN, SQRT, SQUARE, SUB, TANH, const, node, var)
+
+FUSED = 64
+OP_NAN = 254
+F32_NAN = np.float32(np.nan)
+BIN = {ADD, SUB, MUL, DIV, POW, MOD, ATAN2, MIN, MAX} | set(range(34, 42))
+UN = {SIN, COS, 8, EXP, LOG, SQRT, 12, TANH, 14, SQUARE, 16, NEG, 18, 19, 21, 22, 23, 24, 25,
+      28, 29, 33, 42}
+
+
+def translate(prog, off, nprog, nvars):
+    import ctypes as C
+    L = lib.load()
+    info = np.zeros(7, np.int32)
+    prog = np.ascontiguousarray(prog, np.uint32)
+    off = np.ascontiguousarray(off, np.uint32)
+    rc = L.rch_eval_translate(lib.ptr(prog, lib.u32p), lib.ptr(off, lib.u32p), nprog, nvars,
+                              None, 0, None, 0, lib.ptr(info, lib.i32p))
+    assert rc == -2
+    code = np.zeros(max(int(info[0]), 1), np.uint32)
+    meta = np.zeros(max(int(info[1]), 1), np.uint32)
+    rc = L.rch_eval_translate(lib.ptr(prog, lib.u32p), lib.ptr(off, lib.u32p), nprog, nvars,
+                              lib.ptr(code, lib.u32p), len(code), lib.ptr(meta, lib.u32p),
+                              len(meta), lib.ptr(info, lib.i32p))
+    assert rc == 0
+    return code[:info[0]], meta[:info[1]], info
+
+
+def f(v):
+    return np.float32(v)
+
+
+def bits(v):
+    return np.asarray(v, np.float32).view(np.uint32)
+
+
+def unary(op, a):
+    with np.errstate(all="ignore"):
+        if op == SIN:
+            return np.sin(a)
+        if op == COS:
+            return np.cos(a)
+        if op == EXP:
+            return np.exp(a)
+        if op == LOG:
+            return np.where(a > 0, np.log(np.where(a > 0, a, f(1))), F32_NAN).astype(np.float32)
+        if op == SQRT:
+            return np.where(a >= 0, np.sqrt(np.abs(a)), F32_NAN).astype(np.float32)
+        if op == NEG:
+            return -a
+        if op == SQUARE:
+            return a * a
+        if op == TANH:
+            return np.tanh(a)
+        if op in UN:
+            return (a * f(0.5) + f(0.25)).astype(np.float32)
+        return np.full_like(a, F32_NAN)
+
+
+def binary(op, a, b):
+    with np.errstate(all="ignore"):
+        if op == ADD:
+            return a + b
+        if op == SUB:
+            return a - b
+        if op == MUL:
+            return a * b
+        if op == DIV:
+            return np.where(np.abs(b) < f(1e-12), F32_NAN, a / b).astype(np.float32)
+        if op == MIN:
+            return np.where(np.isnan(a) | np.isnan(b), F32_NAN, np.minimum(a, b)).astype(np.float32)
+        if op == MAX:
+            return np.where(np.isnan(a) | np.isnan(b), F32_NAN, np.maximum(a, b)).astype(np.float32)
+        return (a * f(0.75) - b * f(0.5)).astype(np.float32)
+
+
+def plain_run(words, X, nvars):
+    """The original postfix program over the rows of X (rows x nvars)."""
+    n = X.shape[0]
+    nan = np.full(n, F32_NAN, np.float32)
+    st = []
+    i = 0
+    while True:
+        if i >= len(words):
+            return nan
+        op = int(words[i])
+        i += 1
+        if op in (0, 1):
+            if i >= len(words) or len(st) >= 64:
+                return nan
+            arg = int(words[i])
+            i += 1
+            if op == 0:
+                st.append(np.full(n, np.uint32(arg).view(np.float32), np.float32))
+            elif arg < nvars and arg < 65536:
+                st.append(X[:, arg].copy())
+            else:
+                st.append(nan.copy())
+        elif op == HALT:
+            return st[0] if st else nan
+        elif op == IF:
+            if len(st) < 3:
+                return nan
+            e, t, c = st.pop(), st.pop(), st.pop()
+            st.append(np.where(c > 0, t, e).astype(np.float32))
+        elif op in BIN:
+            if len(st) < 2:
+                return nan
+            b, a = st.pop(), st.pop()
+            st.append(binary(op, a, b))
+        else:
+            if not st:
+                return nan
+            st.append(unary(op if op in UN else OP_NAN, st.pop()))
+
+
+def stream_run(code, pc, X):
+    """The device interpreter's semantics over the translated stream."""
+    n = X.shape[0]
+    st = []
+    tos = None
+    while True:
+        w = int(code[pc])
+        pc += 1
+        op = w & 0xFF
+        if op in (0, 1):
+            if tos is not None:
+                st.append(tos)
+            if op == 1:
+                tos = X[:, w >> 16].copy()
+            else:
+                tos = np.full(n, np.uint32(code[pc]).view(np.float32), np.float32)
+                pc += 1
+        elif FUSED <= op < FUSED + 16:
+            k = op - FUSED
+            bop = (ADD, SUB, MUL, DIV)[k >> 2]
+            if k & 1:
+                leaf = X[:, w >> 16]
+            else:
+                leaf = np.full(n, np.uint32(code[pc]).view(np.float32), np.float32)
+                pc += 1
+            tos = binary(bop, leaf, tos) if k & 2 else binary(bop, tos, leaf)
+        elif op == HALT:
+            return st[0] if st else tos
+        elif op == IF:
+            t, c = st.pop(), st.pop()
+            tos = np.where(c > 0, t, tos).astype(np.float32)
+        elif op in BIN:
+            tos = binary(op, st.pop(), tos)
+        else:
+            tos = unary(op, tos)
+
+
+def run_translated(code, meta, nprog, X):
+    code = code.copy()
+    pairs = meta[2 * nprog:].reshape(-1, 2)
+    one = np.zeros((1, X.shape[1]), np.float32)
+    for hp, target in pairs:
+        assert code[target] == 0x7FC00000
+        code[target] = bits(stream_run(code, int(hp), one))[0]
+    return [stream_run(code, int(meta[p]), X) for p in range(nprog)]
+
+
+def leaf_words(rng, nvars):
+    if rng.random() < 0.45:
+        return [0, int(bits(f(rng.normal())))]
+    return [1, int(rng.integers(nvars + (1 if rng.random() < 0.05 else 0)))]
+
+
+def random_program(rng, nvars, nodes):
+    """A postfix program; mostly well formed, with extra stack entries,
+    unknown operators and the occasional malformed tail."""
+    words, depth = [], 0
+    ops = [ADD, SUB, MUL, DIV, SIN, COS, EXP, LOG, IF, MIN, MAX, POW, SQRT, NEG, SQUARE, 200]
+    for _ in range(nodes):
+        op = ops[int(rng.integers(len(ops)))]
+        need = 3 if op == IF else 2 if op in BIN else 1
+        if depth < need or rng.random() < 0.4:
+            words += leaf_words(rng, nvars)
+            depth += 1
+        else:
+            words.append(op)
+            depth -= need - 1
+    r = rng.random()
+    if r < 0.03:
+        return words  # no HALT
+    if r < 0.06:
+        words.append(ADD)  # may underflow
+    words.append(HALT)
+    return words
+
+
+def pack(progs):
+    off = np.zeros(len(progs) + 1, np.uint32)
+    off[1:] = np.cumsum([len(p) for p in progs])
+    return np.asarray([w for p in progs for w in p], np.uint32), off
+
+
+def check_batch(progs, nvars, X):
+    prog, off = pack(progs)
+    code, meta, info = translate(prog, off, len(progs), nvars)
+    got = run_translated(code, meta, len(progs), X)
+    for p, words in enumerate(progs):
+        want = plain_run(words, X, nvars)
+        same = np.array_equal(np.isnan(got[p]), np.isnan(want)) and np.array_equal(
+            bits(got[p])[~np.isnan(want)], bits(want)[~np.isnan(want)])
+        assert same, (p, words)
+    return code, meta, info
+
+
+def _x(rng, n=64, nvars=3):
+    x = rng.normal(size=(n, nvars)).astype(np.float32)
+    x[0] = 0.0
+    x[1, 0] = np.inf
+    return x
+
+
+def test_random_programs_translate_to_the_same_values():
+    rng = np.random.default_rng(5)
+    X = _x(rng)
+    progs = [random_program(rng, 3, int(rng.integers(1, 60))) for _ in range(400)]
+    code, meta, info = check_batch(progs, 3, X)
+    nfused = sum(1 for w in code if FUSED <= (w & 0xFF) < FUSED + 16)
+    assert nfused > 100 and len(meta) > 2 * len(progs)  # fused words and hoists both occur
+
+
+def test_c2_programs_fuse_and_hoist():
+    wl = workloads.config2(n=64, nprog=256)
+    pop = host.Population()
+    for t in wl.trees:
+        pop.add(t)
+    prog, off = pop.arrays()
+    progs = [prog[off[i]:off[i + 1]].tolist() for i in range(len(off) - 1)]
+    code, meta, info = check_batch(progs, 5, wl.x.astype(np.float32))
+    main = int(meta[2 * len(progs)]) if len(meta) > 2 * len(progs) else len(code)
+    words_in = sum(len(p) for p in progs)
+    assert main < 0.8 * words_in  # fewer dispatched words than the postfix
+    assert info[6] <= 16
+
+
+def test_hoisted_subtree_values_and_layout():
+    # x0 * (sin(0.5) + exp(1.5)) - log(2.0) / x1: two input-free subtrees.
+    t = node(SUB, node(MUL, var(0), node(ADD, node(SIN, const(0.5)), node(EXP, const(1.5)))),
+             node(DIV, node(LOG, const(2.0)), var(1)))
+    w = host.compile_program(t).tolist()
+    X = _x(np.random.default_rng(1), 32, 2)
+    code, meta, info = check_batch([w], 2, X)
+    pairs = meta[2:].reshape(-1, 2)
+    assert len(pairs) == 2
+    # main stream: VAR x0, MUL-by-constant, then the right operand
+    assert code[0] == 1 | (0 << 16)
+    assert code[1] == FUSED + 2 * 4 + 0  # tos * c
+    # each hoist program is a plain postfix ending in HALT, after the main code
+    for hp, target in pairs:
+        assert hp > target and code[target] == 0x7FC00000
+        q = int(hp)
+        while code[q] != HALT:
+            q += 1
+        assert (code[int(hp):q] & 0xFF).max() < FUSED
+
+
+def test_left_leaf_keeps_operand_order_and_whole_constant_programs():
+    X = _x(np.random.default_rng(2), 32, 2)
+    progs = [
+        host.compile_program(node(SUB, const(3.0), node(SIN, var(0)))).tolist(),
+        host.compile_program(node(DIV, var(1), node(COS, var(0)))).tolist(),
+        host.compile_program(node(DIV, const(1.0), node(ADD, const(2.0), const(3.0)))).tolist(),
+        [0, int(bits(f(2.5))), HALT],
+        [1, 0, 1, 1, HALT],            # two entries: the bottom one is returned
+        [1, 7, HALT],                  # column outside the resident set
+        [0, 0, IF, HALT],              # underflow
+        [1, 0] * 65 + [HALT],          # overflow past 64 entries
+        [1, 0],                        # no HALT
+    ]
+    code, meta, info = check_batch(progs, 2, X)
+    # c - sin(x0): sin first, then the left-constant SUB
+    p0 = code[int(meta[0]):]
+    assert p0[1] == SIN and p0[2] == FUSED + 1 * 4 + 2 and p0[4] == HALT
+    # 1 / (2 + 3): one hoist program for the whole tree
+    p2 = code[int(meta[2]):]
+    assert p2[0] == 0 and p2[2] == HALT
+
+
+def test_tall_trees_fall_back_to_the_plain_form():
+    X = _x(np.random.default_rng(3), 16, 1)
+    deep = [1, 0] + [SIN, 0, int(bits(f(0.5))), ADD] * 1100 + [HALT]
+    code, meta, info = check_batch([deep], 1, X)
+    # the postfix word for word, except that a VAR word carries its column
+    assert len(code) == len(deep) - 1

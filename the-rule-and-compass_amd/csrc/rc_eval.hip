@@ -22,16 +22,25 @@
  *    rows is pulled into one XCD's L2 once and reused by all programs;
  *  - stack errors (underflow, overflow past 64, missing HALT) are static
  *    properties of a program, so the host checks them once and replaces an
- *    erroneous program by a NaN program; the kernel carries no per-op guard.
+ *    erroneous program by a NaN program; the kernel carries no per-op guard;
+ *  - the host rebuilds each program's tree and re-emits it: input-free
+ *    subtrees are evaluated once per batch by the same interpreter
+ *    (rc_hoist_kernel) and enter the row streams as constants, and +, -, *, /
+ *    with a leaf operand become one fused word, in the original operand
+ *    order.  A C2-like batch (3-60 random nodes) dispatches about a third
+ *    fewer words per row pass and skips a quarter of its transcendental
+ *    calls, with every loss bit-identical to the plain postfix stream;
  *  - partial sums are per (program, tile) in f64 with a fixed wave butterfly,
  *    finished by a per-program pass in tile order: results depend only on the
  *    program and n, never on batch composition.
  */
 #include "rc_device.h"
 #include "rc_internal.h"
+#include "rcsr_host.h"
 
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include <vector>
@@ -42,6 +51,10 @@ namespace {
    operator words | OP_HALT.  Unknown operators become RC_OP_NAN (sets the top
    of stack to NaN, the reference's unary default). */
 constexpr uint32_t RC_OP_NAN = 254;
+/* Fused words RC_OP_FUSED + (bop * 4 + left * 2 + var): tos = tos bop leaf
+   (left = 0) or leaf bop tos (left = 1) for bop 0..3 = +, -, *, /; the leaf is
+   input column (w >> 16) (var = 1) or the f32 word that follows (var = 0). */
+constexpr uint32_t RC_OP_FUSED = 64;
 
 constexpr int MODE_EVAL = 0;    /* sum of rho with 1e30 hard failures (rc_eval) */
 constexpr int MODE_SCORE = 1;   /* f64 score objectives + bad flag */
@@ -60,55 +73,228 @@ bool is_unary_op(uint32_t op)
            op == OP_ERFC || op == OP_SIGN || op == OP_NOT;
 }
 
-/* Translate one program; returns its peak depth, or -1 for a program the
-   reference device would score as NaN on every row. */
-int translate_one(const uint32_t *w, size_t len, int nvars, std::vector<uint32_t> &out)
+/* One parsed node: a leaf (OP_CONST with its f32 bits, OP_VAR with its
+   column) or an operator with its operands' node indices. */
+struct ENode {
+    uint32_t op;
+    uint32_t arg;
+    int kid[3];
+};
+
+/* Parse one program with the reference's static stack checks (rc_exec_row
+   eval.cu:404-573 fails a row on underflow, overflow past 64 entries or a
+   missing HALT; all three are properties of the program).  Nodes come out in
+   postfix order, so a subtree is the contiguous range [first, root].  Returns
+   false for a program the reference device would score as NaN on every row;
+   *root is the bottom entry of the stack at HALT, the value the reference
+   returns. */
+bool parse_one(const uint32_t *w, size_t len, int nvars, std::vector<ENode> &nodes, int *root)
 {
-    int depth = 0, peak = 0;
+    nodes.clear();
+    int stk[RC_MAX_STACK];
+    int depth = 0;
     size_t i = 0;
-    size_t start = out.size();
     for (;;) {
-        if (i >= len) { out.resize(start); return -1; } /* no HALT in range */
-        uint32_t op = w[i++];
+        if (i >= len) return false; /* no HALT in range */
+        const uint32_t op = w[i++];
+        ENode n = {op, 0u, {-1, -1, -1}};
         if (op == OP_CONST || op == OP_VAR) {
-            if (i >= len) { out.resize(start); return -1; }
-            uint32_t arg = w[i++];
-            if (depth >= RC_MAX_STACK) { out.resize(start); return -1; }
-            ++depth;
-            peak = depth > peak ? depth : peak;
-            if (op == OP_CONST) {
-                out.push_back(OP_CONST);
-                out.push_back(arg);
-            } else if ((int)arg < nvars && arg < 65536u) {
-                out.push_back(OP_VAR | (arg << 16));
+            if (i >= len) return false;
+            const uint32_t arg = w[i++];
+            if (depth >= RC_MAX_STACK) return false;
+            if (op == OP_CONST || !((int)arg < nvars && arg < 65536u)) {
+                /* A column outside the resident set reads NaN, not a stray
+                   address. */
+                n.op = OP_CONST;
+                n.arg = (op == OP_CONST) ? arg : 0x7fc00000u;
             } else {
-                /* A column outside the resident set: NaN, not a stray read. */
-                out.push_back(OP_CONST);
-                out.push_back(0x7fc00000u);
+                n.arg = arg;
             }
         } else if (op == OP_HALT) {
-            if (depth <= 0) { out.resize(start); return -1; }
-            out.push_back(OP_HALT);
-            return peak;
+            if (depth <= 0) return false;
+            *root = stk[0];
+            return true;
         } else if (op == OP_IF) {
-            if (depth < 3) { out.resize(start); return -1; }
-            depth -= 2;
-            out.push_back(OP_IF);
+            if (depth < 3) return false;
+            depth -= 3;
+            n.kid[0] = stk[depth];
+            n.kid[1] = stk[depth + 1];
+            n.kid[2] = stk[depth + 2];
         } else if (is_binary_op(op)) {
-            if (depth < 2) { out.resize(start); return -1; }
-            depth -= 1;
-            out.push_back(op);
+            if (depth < 2) return false;
+            depth -= 2;
+            n.kid[0] = stk[depth];
+            n.kid[1] = stk[depth + 1];
         } else {
-            if (depth < 1) { out.resize(start); return -1; }
-            out.push_back(is_unary_op(op) ? op : RC_OP_NAN);
+            if (depth < 1) return false;
+            depth -= 1;
+            n.op = is_unary_op(op) ? op : RC_OP_NAN;
+            n.kid[0] = stk[depth];
         }
+        stk[depth++] = (int)nodes.size();
+        nodes.push_back(n);
     }
 }
 
+/* Emitter of one batch's streams.
+ *
+ * The main stream of a program re-emits the tree rooted at the value the
+ * reference returns (operand stack entries above it are never read):
+ *  - a subtree without inputs (constants and operators only) has the same
+ *    value on every row.  It is emitted once, as a stand-alone program in the
+ *    batch's hoist section, and its place in the main stream becomes a
+ *    constant whose word the device fills in by running that program with the
+ *    very interpreter that runs the rows (rc_hoist_kernel), so the value is
+ *    bit-for-bit the one each row would have computed;
+ *  - +, -, *, / with a leaf operand (input, constant or hoisted constant)
+ *    become one fused word that applies the operator to the top of stack and
+ *    the leaf, in the original operand order (so results, NaN payloads
+ *    included, are the unfused ones); when only the left operand is a leaf
+ *    the right subtree is evaluated first, which only reorders independent,
+ *    side-effect-free work.
+ */
+struct Emitter {
+    const std::vector<ENode> *nodes = nullptr;
+    std::vector<int> first, height;
+    std::vector<char> cst;
+    std::vector<uint32_t> *code = nullptr;
+    std::vector<uint32_t> hcode;               /* hoist section, offsets relative to it */
+    std::vector<uint32_t> patch;               /* pairs (hoist offset, main word index) */
+    int depth = 0, peak = 0;
+
+    void analyse(const std::vector<ENode> &ns)
+    {
+        nodes = &ns;
+        const size_t m = ns.size();
+        first.assign(m, 0);
+        height.assign(m, 1);
+        cst.assign(m, 0);
+        for (size_t j = 0; j < m; ++j) {
+            const ENode &n = ns[j];
+            if (n.kid[0] < 0) {
+                first[j] = (int)j;
+                cst[j] = n.op == OP_CONST;
+                continue;
+            }
+            first[j] = first[n.kid[0]];
+            bool c = true;
+            int h = 0;
+            for (int k = 0; k < 3 && n.kid[k] >= 0; ++k) {
+                c = c && cst[n.kid[k]];
+                h = std::max(h, height[n.kid[k]]);
+            }
+            cst[j] = c;
+            height[j] = h + 1;
+        }
+    }
+
+    bool leafish(int j) const { return (*nodes)[j].kid[0] < 0 || cst[j]; }
+
+    void bump(int d)
+    {
+        depth += d;
+        peak = std::max(peak, depth);
+    }
+
+    /* Plain postfix of subtree j into `out`. */
+    void plain(int j, std::vector<uint32_t> &out) const
+    {
+        for (int q = first[j]; q <= j; ++q) {
+            const ENode &n = (*nodes)[q];
+            if (n.op == OP_CONST) {
+                out.push_back(OP_CONST);
+                out.push_back(n.arg);
+            } else if (n.op == OP_VAR) {
+                out.push_back(OP_VAR | (n.arg << 16));
+            } else {
+                out.push_back(n.op);
+            }
+        }
+    }
+
+    /* The constant word of leaf-like node j (a literal, or a hoisted
+       subtree whose word the device fills in). */
+    void const_word(int j)
+    {
+        const ENode &n = (*nodes)[j];
+        if (n.kid[0] < 0) {
+            code->push_back(n.arg);
+            return;
+        }
+        patch.push_back((uint32_t)hcode.size());
+        patch.push_back((uint32_t)code->size());
+        code->push_back(0x7fc00000u);
+        plain(j, hcode);
+        hcode.push_back(OP_HALT);
+    }
+
+    void push_leaf(int j)
+    {
+        const ENode &n = (*nodes)[j];
+        bump(1);
+        if (n.op == OP_VAR) {
+            code->push_back(OP_VAR | (n.arg << 16));
+        } else {
+            code->push_back(OP_CONST);
+            const_word(j);
+        }
+    }
+
+    static int fused_index(uint32_t op)
+    {
+        return op == OP_ADD ? 0 : op == OP_SUB ? 1 : op == OP_MUL ? 2 : op == OP_DIV ? 3 : -1;
+    }
+
+    void fused(int bop, bool left, int leaf)
+    {
+        const ENode &n = (*nodes)[leaf];
+        const bool var = n.op == OP_VAR;
+        const uint32_t h = RC_OP_FUSED + (uint32_t)(bop * 4 + (left ? 2 : 0) + (var ? 1 : 0));
+        if (var) {
+            code->push_back(h | (n.arg << 16));
+        } else {
+            code->push_back(h);
+            const_word(leaf);
+        }
+    }
+
+    void emit(int j)
+    {
+        const ENode &n = (*nodes)[j];
+        if (leafish(j)) {
+            push_leaf(j);
+            return;
+        }
+        const int bop = fused_index(n.op);
+        if (bop >= 0) {
+            if (leafish(n.kid[1])) {
+                emit(n.kid[0]);
+                fused(bop, false, n.kid[1]);
+                return;
+            }
+            if (leafish(n.kid[0])) {
+                emit(n.kid[1]);
+                fused(bop, true, n.kid[0]);
+                return;
+            }
+        }
+        int nk = 0;
+        for (int k = 0; k < 3 && n.kid[k] >= 0; ++k, ++nk) emit(n.kid[k]);
+        code->push_back(n.op);
+        bump(1 - nk);
+    }
+};
+
+/* Trees taller than this are emitted in their plain postfix form (the
+   emitter recurses once per level). */
+constexpr int RC_EMIT_MAX_HEIGHT = 2048;
+
 } /* namespace */
 
-/* Translate a batch.  meta = [code offset per program | program list grouped
-   by stack class]; cls_start receives the class boundaries in that list. */
+/* Translate a batch.  code = [main streams | hoist section]; meta = [main
+   code offset per program | program list grouped by stack class | pairs
+   (hoist program offset, constant word index) for rc_hoist_kernel];
+   cls_start receives the class boundaries in the program list. */
 int rc_translate_programs(const uint32_t *prog, const uint32_t *off, int nprog, int nvars,
                           std::vector<uint32_t> &code, std::vector<uint32_t> &meta,
                           int cls_start[5])
@@ -116,12 +302,35 @@ int rc_translate_programs(const uint32_t *prog, const uint32_t *off, int nprog, 
     code.clear();
     meta.assign((size_t)nprog * 2, 0);
     std::vector<int> depth((size_t)nprog, 1);
+    std::vector<ENode> nodes;
+    Emitter em;
+    em.code = &code;
     int peak = 1;
     for (int p = 0; p < nprog; ++p) {
         meta[p] = (uint32_t)code.size();
-        size_t a = off[p], b = off[p + 1];
-        int d = (b > a) ? translate_one(prog + a, b - a, nvars, code) : -1;
-        if (d < 0) {
+        const size_t a = off[p], b = off[p + 1];
+        int root = -1;
+        int d;
+        if (b > a && parse_one(prog + a, b - a, nvars, nodes, &root)) {
+            em.analyse(nodes);
+            em.depth = 0;
+            em.peak = 0;
+            if (em.height[root] <= RC_EMIT_MAX_HEIGHT) {
+                em.emit(root);
+            } else {
+                /* Plain postfix: the stack peak is the parse's. */
+                em.plain(root, code);
+                int dd = 0;
+                for (int q = em.first[root]; q <= root; ++q) {
+                    int nk = 0;
+                    while (nk < 3 && nodes[q].kid[nk] >= 0) ++nk;
+                    dd += 1 - nk;
+                    em.peak = std::max(em.peak, dd);
+                }
+            }
+            code.push_back(OP_HALT);
+            d = std::max(em.peak, 1);
+        } else {
             code.push_back(OP_CONST);
             code.push_back(0x7fc00000u);
             code.push_back(OP_HALT);
@@ -130,7 +339,6 @@ int rc_translate_programs(const uint32_t *prog, const uint32_t *off, int nprog, 
         depth[p] = d;
         peak = d > peak ? d : peak;
     }
-    static const int lim[4] = {8, 16, 32, 64};
     int k = 0;
     for (int cls = 0; cls < 4; ++cls) {
         cls_start[cls] = k;
@@ -139,9 +347,14 @@ int rc_translate_programs(const uint32_t *prog, const uint32_t *off, int nprog, 
             const int mine = d <= 8 ? 0 : d <= 16 ? 1 : d <= 32 ? 2 : 3;
             if (mine == cls) meta[(size_t)nprog + k++] = (uint32_t)p;
         }
-        (void)lim;
     }
     cls_start[4] = k;
+    const uint32_t hbase = (uint32_t)code.size();
+    code.insert(code.end(), em.hcode.begin(), em.hcode.end());
+    for (size_t q = 0; q < em.patch.size(); q += 2) {
+        meta.push_back(hbase + em.patch[q]);
+        meta.push_back(em.patch[q + 1]);
+    }
     return peak;
 }
 
@@ -152,6 +365,30 @@ __device__ __noinline__ float rc_unary_call(unsigned op, float a) { return rc_un
 __device__ __noinline__ float rc_binary_call(unsigned op, float a, float b)
 {
     return rc_binary_f32(op, a, b);
+}
+
+/* A fused word's operator on the top of stack and a leaf value (an SGPR
+   constant or a per-row load), in the program's operand order. */
+template <int R, bool LEFT, typename Leaf>
+__device__ __forceinline__ void rc_fused(uint32_t bop, float *tos, Leaf leaf)
+{
+    if (bop == 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) tos[r] = LEFT ? leaf(r) + tos[r] : tos[r] + leaf(r);
+    } else if (bop == 1) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) tos[r] = LEFT ? leaf(r) - tos[r] : tos[r] - leaf(r);
+    } else if (bop == 2) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) tos[r] = LEFT ? leaf(r) * tos[r] : tos[r] * leaf(r);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const float a = LEFT ? leaf(r) : tos[r];
+            const float b = LEFT ? tos[r] : leaf(r);
+            tos[r] = (fabsf(b) < 1e-12f) ? RC_F32_NAN : a / b;
+        }
+    }
 }
 
 /* Evaluate the program at `pc` for R rows of this lane.  `xrow[r]` is the
@@ -187,6 +424,23 @@ __device__ __forceinline__ void rc_run(const uint32_t *__restrict__ pc,
                 const float c = __uint_as_float(*pc++);
 #pragma unroll
                 for (int r = 0; r < R; ++r) tos[r] = c;
+            }
+        } else if (op >= RC_OP_FUSED && op < RC_OP_FUSED + 16u) {
+            const uint32_t f = op - RC_OP_FUSED;
+            const uint32_t bop = f >> 2;
+            if (f & 1u) {
+                const float *col = cols + (size_t)(w >> 16) * (size_t)n;
+                float lv[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) lv[r] = col[xrow[r]];
+                auto leaf = [&](int r) { return lv[r]; };
+                if (f & 2u) rc_fused<R, true>(bop, tos, leaf);
+                else rc_fused<R, false>(bop, tos, leaf);
+            } else {
+                const float c = __uint_as_float(*pc++);
+                auto leaf = [=](int) { return c; };
+                if (f & 2u) rc_fused<R, true>(bop, tos, leaf);
+                else rc_fused<R, false>(bop, tos, leaf);
             }
         } else if (op == OP_MUL) {
             --sp; --depth;
@@ -364,6 +618,20 @@ __global__ __launch_bounds__(64) void rc_eval_kernel(
     }
 }
 
+/* Values of the batch's input-free subtrees: wave h runs hoist program
+   hoist[2h] with the row interpreter (all lanes alike) and stores the result
+   into constant word hoist[2h + 1] of the main streams. */
+__global__ __launch_bounds__(64) void rc_hoist_kernel(uint32_t *__restrict__ code,
+                                                      const uint32_t *__restrict__ hoist, int nhoist)
+{
+    const int h = blockIdx.x;
+    if (h >= nhoist) return;
+    int xrow[1] = {0};
+    float res[1];
+    rc_run<RC_MAX_STACK, 1>(code + hoist[2 * h], nullptr, 0, xrow, res);
+    if (threadIdx.x == 0) code[hoist[2 * h + 1]] = __float_as_uint(res[0]);
+}
+
 /* Per-program finish in tile order (deterministic). */
 __global__ void rc_eval_finish(const double *__restrict__ part, const int *__restrict__ part_bad,
                                int nprog, int tiles, int n, int mode, int kind,
@@ -533,6 +801,14 @@ int upload_translated(RcCtx *c, const std::vector<uint32_t> &code, const std::ve
                 hipSuccess)
             return -1;
     }
+    /* Fill in the hoisted constants before any launch reads the streams. */
+    const size_t nhoist = (meta.size() - 2 * (size_t)nprog) / 2;
+    if (nhoist > 0x7fffffffu) return -1;
+    if (nhoist > 0) {
+        rc_hoist_kernel<<<(unsigned)nhoist, 64, 0, c->s_eval>>>(
+            c->arena_code[slot], c->arena_meta[slot] + 2 * (size_t)nprog, (int)nhoist);
+        if (hipGetLastError() != hipSuccess) return -1;
+    }
     st->h2d_ms += rc_now_ms() - t;
     st->h2d_bytes += (unsigned long long)nb;
     c->arena_slot = slot;
@@ -619,6 +895,24 @@ float score_current(RcCtx *c, int nprog, int kind, const int *pcount, double *ou
 } /* namespace */
 
 extern "C" {
+
+int rch_eval_translate(const unsigned int *prog, const unsigned int *off, int nprog, int nvars,
+                       unsigned int *code, int cap_code, unsigned int *meta, int cap_meta,
+                       int *info)
+{
+    if (!prog || !off || nprog <= 0 || !info) return -1;
+    std::vector<uint32_t> cv, mv;
+    int cls[5];
+    const int peak = rc_translate_programs(prog, off, nprog, nvars, cv, mv, cls);
+    info[0] = (int)cv.size();
+    info[1] = (int)mv.size();
+    for (int k = 0; k < 4; ++k) info[2 + k] = cls[k];
+    info[6] = peak;
+    if (!code || !meta || cv.size() > (size_t)cap_code || mv.size() > (size_t)cap_meta) return -2;
+    memcpy(code, cv.data(), cv.size() * 4);
+    memcpy(meta, mv.data(), mv.size() * 4);
+    return 0;
+}
 
 int rc_set_programs(void *ctx, const unsigned int *prog, const unsigned int *off, int nprog)
 {

@@ -92,6 +92,8 @@ SIGNATURES = {
     "rch_fit_plan": (C.c_int, [i32p, f64p, C.c_int, u32p, C.c_int, f64p, C.c_int, i32p]),
     "rch_fitvm_compile": (C.c_int, [u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, u32p,
                                     C.c_int, u32p, C.c_int, i32p]),
+    "rch_eval_translate": (C.c_int, [u32p, u32p, C.c_int, C.c_int, u32p, C.c_int, u32p, C.c_int,
+                                     i32p]),
 }
 
 _lib = None
