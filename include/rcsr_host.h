@@ -46,13 +46,16 @@ int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int 
    prog / off / nprog: the rc_set_programs batch; nvars: resident columns.
    code receives [per-program streams | hoist programs], meta receives
    [stream offset per program | program list grouped by stack class | pairs
-   (hoist program offset, constant word to fill)]; info[0..6] = code words,
-   meta words, the four class starts and the batch's peak stack depth (filled
-   even when a buffer is too small).  Returns 0, -1 bad arguments, -2 a
-   buffer is too small. */
+   (hoist program offset, constant word to fill)].  derived (may be NULL: no
+   derived columns) holds *nderived (unary operator, source column) pairs,
+   column nvars + k being pair k; the pairs this batch admits are appended,
+   up to cap_derived pairs in all, and *nderived is updated.  info[0..6] =
+   code words, meta words, the four class starts and the batch's peak stack
+   depth (filled even when a buffer is too small).  Returns 0, -1 bad
+   arguments, -2 a buffer is too small. */
 int rch_eval_translate(const unsigned int *prog, const unsigned int *off, int nprog, int nvars,
                        unsigned int *code, int cap_code, unsigned int *meta, int cap_meta,
-                       int *info);
+                       unsigned int *derived, int cap_derived, int *nderived, int *info);
 
 #ifdef __cplusplus
 }

@@ -27,22 +27,42 @@ UN = {SIN, COS, 8, EXP, LOG, SQRT, 12, TANH, 14, SQUARE, 16, NEG, 18, 19, 21, 22
       28, 29, 33, 42}
 
 
-def translate(prog, off, nprog, nvars):
-    import ctypes as C
+def translate(prog, off, nprog, nvars, derived=(), cap=256):
+    """rch_eval_translate; derived=None disables derived columns, otherwise
+    it lists the (operator, source column) pairs already computed."""
     L = lib.load()
     info = np.zeros(7, np.int32)
     prog = np.ascontiguousarray(prog, np.uint32)
     off = np.ascontiguousarray(off, np.uint32)
+    if derived is None:
+        dv, nd, dcap = None, None, 0
+    else:
+        dv = np.zeros(2 * max(cap, 1), np.uint32)
+        dv[:2 * len(derived)] = np.asarray(derived, np.uint32).reshape(-1)
+        nd = np.asarray([len(derived)], np.int32)
+        dcap = cap
+    dptr = lib.ptr(dv, lib.u32p) if dv is not None else None
+    nptr = lib.ptr(nd, lib.i32p) if nd is not None else None
     rc = L.rch_eval_translate(lib.ptr(prog, lib.u32p), lib.ptr(off, lib.u32p), nprog, nvars,
-                              None, 0, None, 0, lib.ptr(info, lib.i32p))
+                              None, 0, None, 0, dptr, dcap, nptr, lib.ptr(info, lib.i32p))
     assert rc == -2
     code = np.zeros(max(int(info[0]), 1), np.uint32)
     meta = np.zeros(max(int(info[1]), 1), np.uint32)
     rc = L.rch_eval_translate(lib.ptr(prog, lib.u32p), lib.ptr(off, lib.u32p), nprog, nvars,
                               lib.ptr(code, lib.u32p), len(code), lib.ptr(meta, lib.u32p),
-                              len(meta), lib.ptr(info, lib.i32p))
+                              len(meta), dptr, dcap, nptr, lib.ptr(info, lib.i32p))
     assert rc == 0
-    return code[:info[0]], meta[:info[1]], info
+    pairs = [] if dv is None else [(int(dv[2 * k]), int(dv[2 * k + 1])) for k in range(int(nd[0]))]
+    return code[:info[0]], meta[:info[1]], info, pairs
+
+
+def extend_columns(X, pairs):
+    """The derived columns: pair k = (operator, source column) is column
+    nvars + k, computed from the (input or earlier derived) source."""
+    cols = [X[:, j] for j in range(X.shape[1])]
+    for op, src in pairs:
+        cols.append(unary(op, cols[src]))
+    return np.stack(cols, 1) if cols else X
 
 
 def f(v):
@@ -215,16 +235,16 @@ def pack(progs):
     return np.asarray([w for p in progs for w in p], np.uint32), off
 
 
-def check_batch(progs, nvars, X):
+def check_batch(progs, nvars, X, derived=(), cap=256):
     prog, off = pack(progs)
-    code, meta, info = translate(prog, off, len(progs), nvars)
-    got = run_translated(code, meta, len(progs), X)
+    code, meta, info, pairs = translate(prog, off, len(progs), nvars, derived, cap)
+    got = run_translated(code, meta, len(progs), extend_columns(X, pairs))
     for p, words in enumerate(progs):
         want = plain_run(words, X, nvars)
         same = np.array_equal(np.isnan(got[p]), np.isnan(want)) and np.array_equal(
             bits(got[p])[~np.isnan(want)], bits(want)[~np.isnan(want)])
         assert same, (p, words)
-    return code, meta, info
+    return code, meta, info, pairs
 
 
 def _x(rng, n=64, nvars=3):
@@ -238,7 +258,7 @@ def test_random_programs_translate_to_the_same_values():
     rng = np.random.default_rng(5)
     X = _x(rng)
     progs = [random_program(rng, 3, int(rng.integers(1, 60))) for _ in range(400)]
-    code, meta, info = check_batch(progs, 3, X)
+    code, meta, info, pairs = check_batch(progs, 3, X)
     nfused = sum(1 for w in code if FUSED <= (w & 0xFF) < FUSED + 16)
     assert nfused > 100 and len(meta) > 2 * len(progs)  # fused words and hoists both occur
 
@@ -250,7 +270,7 @@ def test_c2_programs_fuse_and_hoist():
         pop.add(t)
     prog, off = pop.arrays()
     progs = [prog[off[i]:off[i + 1]].tolist() for i in range(len(off) - 1)]
-    code, meta, info = check_batch(progs, 5, wl.x.astype(np.float32))
+    code, meta, info, pairs = check_batch(progs, 5, wl.x.astype(np.float32))
     main = int(meta[2 * len(progs)]) if len(meta) > 2 * len(progs) else len(code)
     words_in = sum(len(p) for p in progs)
     assert main < 0.8 * words_in  # fewer dispatched words than the postfix
@@ -263,7 +283,7 @@ def test_hoisted_subtree_values_and_layout():
              node(DIV, node(LOG, const(2.0)), var(1)))
     w = host.compile_program(t).tolist()
     X = _x(np.random.default_rng(1), 32, 2)
-    code, meta, info = check_batch([w], 2, X)
+    code, meta, info, pairs = check_batch([w], 2, X)
     pairs = meta[2:].reshape(-1, 2)
     assert len(pairs) == 2
     # main stream: VAR x0, MUL-by-constant, then the right operand
@@ -291,7 +311,7 @@ def test_left_leaf_keeps_operand_order_and_whole_constant_programs():
         [1, 0] * 65 + [HALT],          # overflow past 64 entries
         [1, 0],                        # no HALT
     ]
-    code, meta, info = check_batch(progs, 2, X)
+    code, meta, info, pairs = check_batch(progs, 2, X)
     # c - sin(x0): sin first, then the left-constant SUB
     p0 = code[int(meta[0]):]
     assert p0[1] == SIN and p0[2] == FUSED + 1 * 4 + 2 and p0[4] == HALT
@@ -303,6 +323,40 @@ def test_left_leaf_keeps_operand_order_and_whole_constant_programs():
 def test_tall_trees_fall_back_to_the_plain_form():
     X = _x(np.random.default_rng(3), 16, 1)
     deep = [1, 0] + [SIN, 0, int(bits(f(0.5))), ADD] * 1100 + [HALT]
-    code, meta, info = check_batch([deep], 1, X)
+    code, meta, info, pairs = check_batch([deep], 1, X)
     # the postfix word for word, except that a VAR word carries its column
     assert len(code) == len(deep) - 1
+
+
+def test_derived_columns_are_shared_kept_and_capped():
+    X = _x(np.random.default_rng(4), 48, 3)
+    sx0 = node(SIN, var(0))
+    a = [node(ADD, node(MUL, sx0, const(2.0)), var(1)),
+         node(SUB, sx0, node(COS, var(1))),
+         node(EXP, sx0),
+         node(DIV, node(EXP, sx0), var(2)),
+         node(LOG, var(2))]
+    progs = [host.compile_program(t).tolist() for t in a]
+    code, meta, info, pairs = check_batch(progs, 3, X)
+    assert pairs[0] == (SIN, 0)                  # four uses
+    assert (EXP, 3) in pairs                     # exp of the derived sin(x0), twice
+    assert (COS, 1) not in pairs and (LOG, 2) not in pairs  # one use each
+    # the first program reads column 3 and multiplies it by the constant
+    p0 = code[int(meta[0]):]
+    assert p0[0] == 1 | (3 << 16) and p0[1] == FUSED + 2 * 4 + 0
+    # a later batch reuses the computed columns and appends only new pairs
+    b = [node(ADD, node(COS, var(1)), node(MUL, node(COS, var(1)), var(0))),
+         node(MUL, sx0, var(2))]
+    progs2 = [host.compile_program(t).tolist() for t in b]
+    _, _, _, pairs2 = check_batch(progs2, 3, X, derived=pairs)
+    assert pairs2[:len(pairs)] == pairs and pairs2[len(pairs):] == [(COS, 1)]
+    # without the table: plain input reads, same values
+    _, _, _, none = check_batch(progs, 3, X, derived=None)
+    assert none == []
+    # the cap holds
+    rng = np.random.default_rng(6)
+    many = [random_program(rng, 3, int(rng.integers(5, 60))) for _ in range(300)]
+    _, _, _, capped = check_batch(many, 3, X, cap=3)
+    assert len(capped) == 3
+    _, _, _, free = check_batch(many, 3, X)
+    assert len(free) > 3

@@ -216,6 +216,41 @@ def test_pool_api_matches_packed_batch_and_hash_is_fnv1a(data):
         assert int(h[i]) == hv
 
 
+def _translated_depth(tree):
+    from rcsr_hip import lib
+    w = np.ascontiguousarray(host.compile_program(tree), np.uint32)
+    off = np.asarray([0, len(w)], np.uint32)
+    info = np.zeros(7, np.int32)
+    L = lib.load()
+    L.rch_eval_translate(lib.ptr(w, lib.u32p), lib.ptr(off, lib.u32p), 1, 2, None, 0, None, 0,
+                         None, 0, None, lib.ptr(info, lib.i32p))
+    return int(info[6])
+
+
+def test_derived_columns_match_inline_evaluation(data):
+    """Unary-of-input subtrees shared by a batch read derived columns
+    computed once; each program alone (no pair used twice) evaluates them
+    inline.  Predictions must agree bit for bit (NaN positions included)."""
+    x, y, ev = data
+    n, v = x.shape
+    cols = x.T.astype(np.float32).reshape(-1)
+    s0, e1, l2 = node(SIN, var(0)), node(EXP, node(COS, var(1))), node(LOG, var(2))
+    trees = [node(MUL, s0, const(1.5)), node(ADD, s0, e1), node(DIV, e1, node(SQRT, l2)),
+             node(SUB, l2, node(MUL, s0, var(1))), node(TANH, l2), node(SQUARE, node(TANH, l2)),
+             node(ADD, node(EXP, node(COS, var(1))), node(SIN, var(0)))]
+    batch = ev.predict(trees)
+    for t, p in zip(trees, batch):
+        one = host.GpuEvaluator(cols, n, v, y.astype(np.float32))
+        q = one.predict([t])[0]
+        one.close()
+        assert np.array_equal(np.isnan(p), np.isnan(q)), str(t)
+        fin = ~np.isnan(q)
+        assert np.array_equal(p[fin].view(np.uint32), q[fin].view(np.uint32)), str(t)
+        want = oracle.predict(t, x)
+        ok = np.isfinite(want)
+        assert np.allclose(p[ok], want[ok], rtol=1e-5, atol=1e-6), str(t)
+
+
 def test_large_batch_row_tiling_is_exact_at_tile_edges():
     """n not a multiple of the 4096-row tile, programs of every stack class."""
     rng = np.random.default_rng(3)
@@ -223,13 +258,17 @@ def test_large_batch_row_tiling_is_exact_at_tile_edges():
     x = rng.normal(size=(n, 2)).astype(np.float32).astype(np.float64)
     y = rng.normal(size=n).astype(np.float32).astype(np.float64)
     ev = host.GpuEvaluator(x.T.astype(np.float32).reshape(-1), n, 2, y.astype(np.float32))
-    deep = var(0)
+    # Right-nested chains whose left operands are operator nodes keep one
+    # stack entry per level through the translation (a leaf operand would be
+    # fused instead).
+    deep = node(MUL, var(0), var(1))
     for i in range(20):
-        deep = node(ADD, var(i % 2), deep)       # stack depth 21 -> the 32-slot class
-    mid = var(1)
+        deep = node(ADD, node(MUL, var(i % 2), var(1 - i % 2)), deep)   # depth 21: 32-slot class
+    mid = node(ADD, var(1), const(0.5))
     for i in range(12):
-        mid = node(MUL, const(1.0 + 0.01 * i), node(ADD, var(0), mid))  # depth ~13
+        mid = node(MUL, node(ADD, var(0), const(1.0 + 0.01 * i)), mid)  # depth 13: 16-slot class
     trees = [node(ADD, var(0), var(1)), mid, deep]
+    assert [_translated_depth(t) for t in trees] == [1, 13, 21]
     pred = ev.predict(trees)
     for t, p in zip(trees, pred):
         want = oracle.predict(t, x)

@@ -160,6 +160,7 @@ void *rc_create(const float *cols, int n, int v, const float *y)
     if (hipStreamCreateWithFlags(&c->s_eval, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->s_fit, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess ||
+        hipEventCreate(&c->tp) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming) != hipSuccess) {
         (void)hipGetLastError();
         rc_rows_free(&c->sets[0]);
@@ -188,6 +189,7 @@ void rc_destroy(void *ctx)
     if (c->d_score) (void)hipFree(c->d_score);
     if (c->d_status) (void)hipFree(c->d_status);
     if (c->d_hash) (void)hipFree(c->d_hash);
+    if (c->d_derived) (void)hipFree(c->d_derived);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->h_out) (void)hipHostFree(c->h_out);
     rc_fit_state_free(c);
@@ -195,6 +197,7 @@ void rc_destroy(void *ctx)
     if (c->s_fit) (void)hipStreamDestroy(c->s_fit);
     if (c->t0) (void)hipEventDestroy(c->t0);
     if (c->t1) (void)hipEventDestroy(c->t1);
+    if (c->tp) (void)hipEventDestroy(c->tp);
     if (c->ev_done) (void)hipEventDestroy(c->ev_done);
     delete c;
 }

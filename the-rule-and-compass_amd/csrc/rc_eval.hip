@@ -42,6 +42,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <unordered_map>
 
 #include <vector>
 
@@ -146,16 +147,23 @@ bool parse_one(const uint32_t *w, size_t len, int nvars, std::vector<ENode> &nod
  *    constant whose word the device fills in by running that program with the
  *    very interpreter that runs the rows (rc_hoist_kernel), so the value is
  *    bit-for-bit the one each row would have computed;
- *  - +, -, *, / with a leaf operand (input, constant or hoisted constant)
- *    become one fused word that applies the operator to the top of stack and
- *    the leaf, in the original operand order (so results, NaN payloads
- *    included, are the unfused ones); when only the left operand is a leaf
- *    the right subtree is evaluated first, which only reorders independent,
- *    side-effect-free work.
+ *  - a unary operator applied to an input column (or to such a derived
+ *    column) recurs across the programs of a batch, e.g. sin(x1) hundreds of
+ *    times in 4096 random programs over 5 inputs.  The pairs (operator,
+ *    column) used at least twice become derived columns, computed once over
+ *    the rows by the interpreter's own unary code (rc_derive_kernel) and kept
+ *    for the evaluator's lifetime (the f32 rows never change); the node then
+ *    reads that column like an input;
+ *  - +, -, *, / with a leaf operand (input or derived column, constant,
+ *    hoisted constant) become one fused word that applies the operator to the
+ *    top of stack and the leaf, in the original operand order (so results,
+ *    NaN payloads included, are the unfused ones); when only the left operand
+ *    is a leaf the right subtree is evaluated first, which only reorders
+ *    independent, side-effect-free work.
  */
 struct Emitter {
     const std::vector<ENode> *nodes = nullptr;
-    std::vector<int> first, height;
+    std::vector<int> first, height, colid;
     std::vector<char> cst;
     std::vector<uint32_t> *code = nullptr;
     std::vector<uint32_t> hcode;               /* hoist section, offsets relative to it */
@@ -169,11 +177,13 @@ struct Emitter {
         first.assign(m, 0);
         height.assign(m, 1);
         cst.assign(m, 0);
+        colid.assign(m, -1);
         for (size_t j = 0; j < m; ++j) {
             const ENode &n = ns[j];
             if (n.kid[0] < 0) {
                 first[j] = (int)j;
                 cst[j] = n.op == OP_CONST;
+                if (n.op == OP_VAR) colid[j] = (int)n.arg;
                 continue;
             }
             first[j] = first[n.kid[0]];
@@ -188,7 +198,7 @@ struct Emitter {
         }
     }
 
-    bool leafish(int j) const { return (*nodes)[j].kid[0] < 0 || cst[j]; }
+    bool leafish(int j) const { return (*nodes)[j].kid[0] < 0 || cst[j] || colid[j] >= 0; }
 
     void bump(int d)
     {
@@ -212,7 +222,7 @@ struct Emitter {
         }
     }
 
-    /* The constant word of leaf-like node j (a literal, or a hoisted
+    /* The constant word of constant-valued node j (a literal, or a hoisted
        subtree whose word the device fills in). */
     void const_word(int j)
     {
@@ -230,10 +240,9 @@ struct Emitter {
 
     void push_leaf(int j)
     {
-        const ENode &n = (*nodes)[j];
         bump(1);
-        if (n.op == OP_VAR) {
-            code->push_back(OP_VAR | (n.arg << 16));
+        if (colid[j] >= 0) {
+            code->push_back(OP_VAR | ((uint32_t)colid[j] << 16));
         } else {
             code->push_back(OP_CONST);
             const_word(j);
@@ -247,11 +256,10 @@ struct Emitter {
 
     void fused(int bop, bool left, int leaf)
     {
-        const ENode &n = (*nodes)[leaf];
-        const bool var = n.op == OP_VAR;
+        const bool var = colid[leaf] >= 0;
         const uint32_t h = RC_OP_FUSED + (uint32_t)(bop * 4 + (left ? 2 : 0) + (var ? 1 : 0));
         if (var) {
-            code->push_back(h | (n.arg << 16));
+            code->push_back(h | ((uint32_t)colid[leaf] << 16));
         } else {
             code->push_back(h);
             const_word(leaf);
@@ -289,30 +297,100 @@ struct Emitter {
    emitter recurses once per level). */
 constexpr int RC_EMIT_MAX_HEIGHT = 2048;
 
+/* Derived columns: passes of (assign known pairs, admit new ones). */
+constexpr int RC_DERIVE_PASSES = 4;
+
+uint64_t derive_key(uint32_t op, int col) { return ((uint64_t)op << 32) | (uint32_t)col; }
+
+/* Mark the unary-of-column nodes of the live trees with their derived
+   columns, admitting pairs used at least twice (most used first) up to
+   max_derived columns in all.  derived holds (operator, source column) per
+   derived column; column nvars + k is pair k. */
+void assign_derived(Emitter &em, const std::vector<ENode> &all, const std::vector<int> &pstart,
+                    const std::vector<int> &proot, int nvars, std::vector<uint32_t> &derived,
+                    int max_derived)
+{
+    std::unordered_map<uint64_t, int> have;
+    for (size_t k = 0; k + 1 < derived.size(); k += 2)
+        have[derive_key(derived[k], (int)derived[k + 1])] = nvars + (int)(k / 2);
+    for (int pass = 0; pass < RC_DERIVE_PASSES; ++pass) {
+        std::unordered_map<uint64_t, int> count;
+        for (size_t p = 0; p + 1 < pstart.size(); ++p) {
+            if (proot[p] < 0) continue;
+            for (int j = pstart[p]; j <= proot[p]; ++j) {
+                const ENode &n = all[j];
+                if (n.kid[0] < 0 || n.kid[1] >= 0 || n.op == RC_OP_NAN || em.colid[j] >= 0) continue;
+                const int src = em.colid[n.kid[0]];
+                if (src < 0) continue;
+                const uint64_t key = derive_key(n.op, src);
+                auto it = have.find(key);
+                if (it != have.end()) em.colid[j] = it->second;
+                else ++count[key];
+            }
+        }
+        if (pass == RC_DERIVE_PASSES - 1) break;
+        std::vector<std::pair<int, uint64_t>> cand;
+        for (const auto &kv : count)
+            if (kv.second >= 2) cand.push_back({-kv.second, kv.first});
+        std::sort(cand.begin(), cand.end());
+        int admitted = 0;
+        for (const auto &c : cand) {
+            if ((int)(derived.size() / 2) >= max_derived) break;
+            have[c.second] = nvars + (int)(derived.size() / 2);
+            derived.push_back((uint32_t)(c.second >> 32));
+            derived.push_back((uint32_t)(c.second & 0xffffffffu));
+            ++admitted;
+        }
+        if (admitted == 0) break;
+    }
+}
+
 } /* namespace */
 
 /* Translate a batch.  code = [main streams | hoist section]; meta = [main
    code offset per program | program list grouped by stack class | pairs
    (hoist program offset, constant word index) for rc_hoist_kernel];
-   cls_start receives the class boundaries in the program list. */
+   cls_start receives the class boundaries in the program list.  derived
+   (when given) holds the derived columns already computed, as (operator,
+   source column) pairs for columns nvars, nvars + 1, ...; pairs the batch
+   admits are appended and must be computed before the streams run. */
 int rc_translate_programs(const uint32_t *prog, const uint32_t *off, int nprog, int nvars,
                           std::vector<uint32_t> &code, std::vector<uint32_t> &meta,
-                          int cls_start[5])
+                          int cls_start[5], std::vector<uint32_t> *derived, int max_derived)
 {
     code.clear();
     meta.assign((size_t)nprog * 2, 0);
-    std::vector<int> depth((size_t)nprog, 1);
-    std::vector<ENode> nodes;
+    std::vector<ENode> all, one;
+    std::vector<int> pstart((size_t)nprog + 1, 0), proot((size_t)nprog, -1);
+    for (int p = 0; p < nprog; ++p) {
+        pstart[p] = (int)all.size();
+        const size_t a = off[p], b = off[p + 1];
+        int root = -1;
+        if (b > a && parse_one(prog + a, b - a, nvars, one, &root)) {
+            const int base = pstart[p];
+            for (ENode n : one) {
+                for (int k = 0; k < 3; ++k)
+                    if (n.kid[k] >= 0) n.kid[k] += base;
+                all.push_back(n);
+            }
+            proot[p] = base + root;
+        }
+    }
+    pstart[nprog] = (int)all.size();
     Emitter em;
     em.code = &code;
+    em.analyse(all);
+    if (derived) {
+        const int cap = std::min(max_derived, 65535 - nvars);
+        assign_derived(em, all, pstart, proot, nvars, *derived, cap);
+    }
+    std::vector<int> depth((size_t)nprog, 1);
     int peak = 1;
     for (int p = 0; p < nprog; ++p) {
         meta[p] = (uint32_t)code.size();
-        const size_t a = off[p], b = off[p + 1];
-        int root = -1;
+        const int root = proot[p];
         int d;
-        if (b > a && parse_one(prog + a, b - a, nvars, nodes, &root)) {
-            em.analyse(nodes);
+        if (root >= 0) {
             em.depth = 0;
             em.peak = 0;
             if (em.height[root] <= RC_EMIT_MAX_HEIGHT) {
@@ -323,7 +401,7 @@ int rc_translate_programs(const uint32_t *prog, const uint32_t *off, int nprog, 
                 int dd = 0;
                 for (int q = em.first[root]; q <= root; ++q) {
                     int nk = 0;
-                    while (nk < 3 && nodes[q].kid[nk] >= 0) ++nk;
+                    while (nk < 3 && all[q].kid[nk] >= 0) ++nk;
                     dd += 1 - nk;
                     em.peak = std::max(em.peak, dd);
                 }
@@ -393,9 +471,17 @@ __device__ __forceinline__ void rc_fused(uint32_t bop, float *tos, Leaf leaf)
 
 /* Evaluate the program at `pc` for R rows of this lane.  `xrow[r]` is the
    row index (clamped in range by the caller). */
+/* Column c of the evaluator's rows: an input (c < nv) or a derived column. */
+__device__ __forceinline__ const float *rc_col(const float *cols, const float *dcols, int n, int nv,
+                                               uint32_t c)
+{
+    return (c < (uint32_t)nv) ? cols + (size_t)c * (size_t)n : dcols + (size_t)(c - nv) * (size_t)n;
+}
+
 template <int S, int R>
 __device__ __forceinline__ void rc_run(const uint32_t *__restrict__ pc,
-                                       const float *__restrict__ cols, int n,
+                                       const float *__restrict__ cols,
+                                       const float *__restrict__ dcols, int n, int nv,
                                        const int *xrow, float *res)
 {
     /* st[r][d]: the compiler keeps this in VGPRs (indexed by the uniform
@@ -417,7 +503,7 @@ __device__ __forceinline__ void rc_run(const uint32_t *__restrict__ pc,
             }
             ++depth;
             if (op == OP_VAR) {
-                const float *col = cols + (size_t)(w >> 16) * (size_t)n;
+                const float *col = rc_col(cols, dcols, n, nv, w >> 16);
 #pragma unroll
                 for (int r = 0; r < R; ++r) tos[r] = col[xrow[r]];
             } else {
@@ -429,7 +515,7 @@ __device__ __forceinline__ void rc_run(const uint32_t *__restrict__ pc,
             const uint32_t f = op - RC_OP_FUSED;
             const uint32_t bop = f >> 2;
             if (f & 1u) {
-                const float *col = cols + (size_t)(w >> 16) * (size_t)n;
+                const float *col = rc_col(cols, dcols, n, nv, w >> 16);
                 float lv[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r) lv[r] = col[xrow[r]];
@@ -528,7 +614,8 @@ __device__ __forceinline__ bool rc_item_of(int g, int nprog, int tiles, int *p, 
 
 template <int S, int R, int MODE>
 __global__ __launch_bounds__(64) void rc_eval_kernel(
-    const float *__restrict__ cols, const float *__restrict__ y, int n,
+    const float *__restrict__ cols, const float *__restrict__ dcols, int nv,
+    const float *__restrict__ y, int n,
     const uint32_t *__restrict__ code, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ plist, int ncls, int tiles, int tile_rows, int kind,
     double *__restrict__ part, int *__restrict__ part_bad, float *__restrict__ pred_out)
@@ -552,7 +639,7 @@ __global__ __launch_bounds__(64) void rc_eval_kernel(
             xrow[r] = ok[r] ? row : row0;
         }
         float res[R];
-        rc_run<S, R>(pc, cols, n, xrow, res);
+        rc_run<S, R>(pc, cols, dcols, n, nv, xrow, res);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (!ok[r]) continue;
@@ -628,8 +715,21 @@ __global__ __launch_bounds__(64) void rc_hoist_kernel(uint32_t *__restrict__ cod
     if (h >= nhoist) return;
     int xrow[1] = {0};
     float res[1];
-    rc_run<RC_MAX_STACK, 1>(code + hoist[2 * h], nullptr, 0, xrow, res);
+    rc_run<RC_MAX_STACK, 1>(code + hoist[2 * h], nullptr, nullptr, 0, 0, xrow, res);
     if (threadIdx.x == 0) code[hoist[2 * h + 1]] = __float_as_uint(res[0]);
+}
+
+/* Derived column dst = op(column src) over the rows, with the operator code
+   the interpreter's unary handlers use (rc_unary_f32 is the same arithmetic
+   as its inline exp / log / sin / cos / neg / square). */
+__global__ __launch_bounds__(256) void rc_derive_kernel(const float *__restrict__ cols,
+                                                        float *__restrict__ dcols, int n, int nv,
+                                                        uint32_t op, int src, int dst)
+{
+    const int row = blockIdx.x * 256 + threadIdx.x;
+    if (row >= n) return;
+    const float *s = rc_col(cols, dcols, n, nv, (uint32_t)src);
+    dcols[(size_t)dst * (size_t)n + (size_t)row] = rc_unary_f32(op, s[row]);
 }
 
 /* Per-program finish in tile order (deterministic). */
@@ -697,20 +797,23 @@ int tile_rows_for(int n, int tuned)
 }
 
 template <int S, int R>
-void launch_class(int mode, dim3 grid, hipStream_t s, const float *cols, const float *y, int n,
-                  const uint32_t *code, const uint32_t *meta, const uint32_t *plist, int ncls,
-                  int tiles, int tile_rows, int kind, double *part, int *part_bad, float *pred)
+void launch_class(int mode, dim3 grid, hipStream_t s, const RcCtx *c, const uint32_t *code,
+                  const uint32_t *meta, const uint32_t *plist, int ncls, int tiles, int tile_rows,
+                  int kind, float *pred)
 {
+    const RcRows *r = &c->sets[0];
     if (mode == MODE_EVAL)
-        rc_eval_kernel<S, R, MODE_EVAL><<<grid, 64, 0, s>>>(cols, y, n, code, meta, plist, ncls, tiles,
-                                                            tile_rows, kind, part, part_bad, pred);
+        rc_eval_kernel<S, R, MODE_EVAL><<<grid, 64, 0, s>>>(
+            r->cols, c->d_derived, r->v, r->y, r->n, code, meta, plist, ncls, tiles, tile_rows, kind,
+            c->d_part, c->d_part_bad, pred);
     else if (mode == MODE_SCORE)
-        rc_eval_kernel<S, R, MODE_SCORE><<<grid, 64, 0, s>>>(cols, y, n, code, meta, plist, ncls, tiles,
-                                                             tile_rows, kind, part, part_bad, pred);
+        rc_eval_kernel<S, R, MODE_SCORE><<<grid, 64, 0, s>>>(
+            r->cols, c->d_derived, r->v, r->y, r->n, code, meta, plist, ncls, tiles, tile_rows, kind,
+            c->d_part, c->d_part_bad, pred);
     else
-        rc_eval_kernel<S, R, MODE_PREDICT><<<grid, 64, 0, s>>>(cols, y, n, code, meta, plist, ncls,
-                                                               tiles, tile_rows, kind, part, part_bad,
-                                                               pred);
+        rc_eval_kernel<S, R, MODE_PREDICT><<<grid, 64, 0, s>>>(
+            r->cols, c->d_derived, r->v, r->y, r->n, code, meta, plist, ncls, tiles, tile_rows, kind,
+            c->d_part, c->d_part_bad, pred);
 }
 
 int launch_rows(RcCtx *c, int mode, int kind, int nprog, const uint32_t *code,
@@ -734,17 +837,17 @@ int launch_rows(RcCtx *c, int mode, int kind, int nprog, const uint32_t *code,
         if (grid > 0x7fffffffu) return -1;
         dim3 g((unsigned)grid);
         if (cls == 0)
-            launch_class<8, 4>(mode, g, s, r->cols, r->y, n, code, meta, plist, ncls, tiles, tile_rows,
-                               kind, c->d_part, c->d_part_bad, pred);
+            launch_class<8, 4>(mode, g, s, c, code, meta, plist, ncls, tiles, tile_rows, kind,
+                               pred);
         else if (cls == 1)
-            launch_class<16, 2>(mode, g, s, r->cols, r->y, n, code, meta, plist, ncls, tiles, tile_rows,
-                                kind, c->d_part, c->d_part_bad, pred);
+            launch_class<16, 2>(mode, g, s, c, code, meta, plist, ncls, tiles, tile_rows, kind,
+                               pred);
         else if (cls == 2)
-            launch_class<32, 1>(mode, g, s, r->cols, r->y, n, code, meta, plist, ncls, tiles, tile_rows,
-                                kind, c->d_part, c->d_part_bad, pred);
+            launch_class<32, 1>(mode, g, s, c, code, meta, plist, ncls, tiles, tile_rows, kind,
+                               pred);
         else
-            launch_class<64, 1>(mode, g, s, r->cols, r->y, n, code, meta, plist, ncls, tiles, tile_rows,
-                                kind, c->d_part, c->d_part_bad, pred);
+            launch_class<64, 1>(mode, g, s, c, code, meta, plist, ncls, tiles, tile_rows, kind,
+                               pred);
         if (hipGetLastError() != hipSuccess) return -1;
     }
     if (mode == MODE_PREDICT) return 0;
@@ -801,19 +904,91 @@ int upload_translated(RcCtx *c, const std::vector<uint32_t> &code, const std::ve
                 hipSuccess)
             return -1;
     }
-    /* Fill in the hoisted constants before any launch reads the streams. */
-    const size_t nhoist = (meta.size() - 2 * (size_t)nprog) / 2;
-    if (nhoist > 0x7fffffffu) return -1;
-    if (nhoist > 0) {
-        rc_hoist_kernel<<<(unsigned)nhoist, 64, 0, c->s_eval>>>(
-            c->arena_code[slot], c->arena_meta[slot] + 2 * (size_t)nprog, (int)nhoist);
-        if (hipGetLastError() != hipSuccess) return -1;
-    }
     st->h2d_ms += rc_now_ms() - t;
     st->h2d_bytes += (unsigned long long)nb;
     c->arena_slot = slot;
     c->nprog = nprog;
     st->batch = nprog;
+    return 0;
+}
+
+/* Derived columns: at most this many, and at most RC_DERIVED_BYTES of them. */
+constexpr int RC_DERIVED_MAX = 256;
+constexpr size_t RC_DERIVED_BYTES = (size_t)2 << 30;
+
+int derived_limit(const RcCtx *c)
+{
+    const size_t n = (size_t)c->sets[0].n;
+    if (n == 0) return 0;
+    return (int)std::min<size_t>(RC_DERIVED_MAX, RC_DERIVED_BYTES / (4 * n));
+}
+
+/* Compute the derived columns the last translation admitted (pairs past
+   c->derived_cols), growing the column block when needed. */
+int derive_columns(RcCtx *c, const std::vector<uint32_t> &derived)
+{
+    RcRows *r = &c->sets[0];
+    const int want = (int)(derived.size() / 2);
+    const size_t colb = sizeof(float) * (size_t)r->n;
+    if (want > c->derived_cap) {
+        const int cap = std::min(derived_limit(c), ((want + 31) / 32) * 32);
+        if (cap < want) return -1;
+        float *nb = nullptr;
+        if (rc_dev_alloc(c, (void **)&nb, colb * (size_t)cap) != 0) return -1;
+        if (c->derived_cols > 0 &&
+            hipMemcpyAsync(nb, c->d_derived, colb * (size_t)c->derived_cols, hipMemcpyDeviceToDevice,
+                           c->s_eval) != hipSuccess) {
+            rc_dev_free(c, nb);
+            return -1;
+        }
+        /* A batch still in flight may read the old block. */
+        if (hipStreamSynchronize(c->s_eval) != hipSuccess) return -1;
+        if (c->d_derived) rc_dev_free(c, c->d_derived);
+        c->d_derived = nb;
+        c->derived_cap = cap;
+    }
+    const unsigned blocks = (unsigned)((r->n + 255) / 256);
+    for (int k = c->derived_cols; k < want; ++k) {
+        rc_derive_kernel<<<blocks, 256, 0, c->s_eval>>>(r->cols, c->d_derived, r->n, r->v,
+                                                        derived[2 * k], (int)derived[2 * k + 1], k);
+        if (hipGetLastError() != hipSuccess) return -1;
+    }
+    c->derived = derived;
+    c->derived_cols = want;
+    return 0;
+}
+
+/* Translate a batch, upload its streams, compute the derived columns it
+   admits and fill in its hoisted constants.  The device work after the
+   upload is timed with the batch's first evaluation (event tp). */
+int translate_and_upload(RcCtx *c, const uint32_t *prog, const uint32_t *off, int nprog)
+{
+    std::vector<uint32_t> code, meta;
+    std::vector<uint32_t> derived = c->derived;
+    const int nv = c->sets[0].v;
+    c->prep_timed = 0;
+    c->max_stack = rc_translate_programs(prog, off, nprog, nv, code, meta, c->cls_start, &derived,
+                                         derived_limit(c));
+    if (upload_translated(c, code, meta, nprog) != 0) return -1;
+    (void)hipEventRecord(c->tp, c->s_eval);
+    if (derived.size() > c->derived.size() && derive_columns(c, derived) != 0) {
+        /* No room for new columns: translate again with the computed ones. */
+        (void)hipGetLastError();
+        derived = c->derived;
+        c->max_stack = rc_translate_programs(prog, off, nprog, nv, code, meta, c->cls_start,
+                                             &derived, c->derived_cols);
+        if (upload_translated(c, code, meta, nprog) != 0) return -1;
+    }
+    /* Fill in the hoisted constants before any launch reads the streams. */
+    const size_t nhoist = (meta.size() - 2 * (size_t)nprog) / 2;
+    if (nhoist > 0x7fffffffu) return -1;
+    if (nhoist > 0) {
+        const int slot = c->arena_slot;
+        rc_hoist_kernel<<<(unsigned)nhoist, 64, 0, c->s_eval>>>(
+            c->arena_code[slot], c->arena_meta[slot] + 2 * (size_t)nprog, (int)nhoist);
+        if (hipGetLastError() != hipSuccess) return -1;
+    }
+    c->prep_timed = 1;
     return 0;
 }
 
@@ -883,7 +1058,8 @@ float score_current(RcCtx *c, int nprog, int kind, const int *pcount, double *ou
         }
     }
     float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, c->t0, c->t1);
+    (void)hipEventElapsedTime(&ms, c->prep_timed ? c->tp : c->t0, c->t1);
+    c->prep_timed = 0;
     st->kernel_ms = ms;
     st->batch = nprog;
     st->wall_ms = rc_now_ms() - c->batch_start_ms;
@@ -898,12 +1074,16 @@ extern "C" {
 
 int rch_eval_translate(const unsigned int *prog, const unsigned int *off, int nprog, int nvars,
                        unsigned int *code, int cap_code, unsigned int *meta, int cap_meta,
-                       int *info)
+                       unsigned int *derived, int cap_derived, int *nderived, int *info)
 {
-    if (!prog || !off || nprog <= 0 || !info) return -1;
-    std::vector<uint32_t> cv, mv;
+    if (!prog || !off || nprog <= 0 || !info || (derived && (!nderived || *nderived < 0 ||
+                                                             *nderived > cap_derived)))
+        return -1;
+    std::vector<uint32_t> cv, mv, dv;
+    if (derived) dv.assign(derived, derived + 2 * (size_t)*nderived);
     int cls[5];
-    const int peak = rc_translate_programs(prog, off, nprog, nvars, cv, mv, cls);
+    const int peak = rc_translate_programs(prog, off, nprog, nvars, cv, mv, cls,
+                                           derived ? &dv : nullptr, derived ? cap_derived : 0);
     info[0] = (int)cv.size();
     info[1] = (int)mv.size();
     for (int k = 0; k < 4; ++k) info[2 + k] = cls[k];
@@ -911,6 +1091,10 @@ int rch_eval_translate(const unsigned int *prog, const unsigned int *off, int np
     if (!code || !meta || cv.size() > (size_t)cap_code || mv.size() > (size_t)cap_meta) return -2;
     memcpy(code, cv.data(), cv.size() * 4);
     memcpy(meta, mv.data(), mv.size() * 4);
+    if (derived) {
+        memcpy(derived, dv.data(), dv.size() * 4);
+        *nderived = (int)(dv.size() / 2);
+    }
     return 0;
 }
 
@@ -923,9 +1107,7 @@ int rc_set_programs(void *ctx, const unsigned int *prog, const unsigned int *off
            staging here is synchronised instead, and the overlap is counted. */
         c->overlapped_uploads++;
     }
-    std::vector<uint32_t> code, meta;
-    c->max_stack = rc_translate_programs(prog, off, nprog, c->sets[0].v, code, meta, c->cls_start);
-    if (upload_translated(c, code, meta, nprog) != 0) {
+    if (translate_and_upload(c, prog, off, nprog) != 0) {
         c->stats.failed = 1;
         return -1;
     }
@@ -974,7 +1156,8 @@ float rc_eval_collect(void *ctx, float *out)
     st->d2h_ms += rc_now_ms() - t;
     st->d2h_bytes += (unsigned long long)c->pending_nprog * 4ull;
     float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, c->t0, c->t1);
+    (void)hipEventElapsedTime(&ms, c->prep_timed ? c->tp : c->t0, c->t1);
+    c->prep_timed = 0;
     st->kernel_ms = ms;
     st->batch = c->pending_nprog;
     st->wall_ms = rc_now_ms() - c->batch_start_ms;
@@ -994,6 +1177,7 @@ int rc_predict(void *ctx, const unsigned int *prog, const unsigned int *off, int
     RcCtx *c = (RcCtx *)ctx;
     if (!c || nprog <= 0 || !out) return -1;
     if (rc_set_programs(ctx, prog, off, nprog) != 0) return -1;
+    c->prep_timed = 0;
     const size_t total = (size_t)nprog * (size_t)c->sets[0].n;
     float *d_pred = nullptr;
     if (hipMalloc((void **)&d_pred, sizeof(float) * (total ? total : 1)) != hipSuccess) return -1;
@@ -1055,10 +1239,7 @@ static int pool_translate(RcCtx *c, int which, int nprog, int width)
     if (!c || nprog <= 0 || nprog > c->pool_cap || width != c->pool_width) return -1;
     std::vector<uint32_t> off((size_t)nprog + 1);
     for (int p = 0; p <= nprog; ++p) off[p] = (uint32_t)((size_t)p * width);
-    std::vector<uint32_t> code, meta;
-    c->max_stack = rc_translate_programs(c->pool_host[which & 1].data(), off.data(), nprog,
-                                         c->sets[0].v, code, meta, c->cls_start);
-    return upload_translated(c, code, meta, nprog);
+    return translate_and_upload(c, c->pool_host[which & 1].data(), off.data(), nprog);
 }
 
 float rc_pool_eval(void *ctx, int which, int nprog, int width, float *out, int loss)

@@ -65,6 +65,13 @@ struct RcCtx {
        occupies [cls_start[k], cls_start[k+1]) of the program list stored
        after the code offsets in the meta arena. */
     int cls_start[5] = {0, 0, 0, 0, 0};
+    /* Derived columns of the evaluator's rows (rc_eval.hip): pair k =
+       (unary operator, source column) is column v + k, stored at
+       d_derived + k * n; computed once and kept with the rows. */
+    std::vector<uint32_t> derived;
+    float *d_derived = nullptr;
+    int derived_cols = 0;   /* computed */
+    int derived_cap = 0;    /* allocated */
     double *d_part = nullptr; /* per (program, tile) partial sums */
     size_t part_cap = 0;
     int *d_part_bad = nullptr;
@@ -89,6 +96,11 @@ struct RcCtx {
     hipStream_t s_eval = nullptr;
     hipStream_t s_fit = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
+    /* Recorded after a batch's upload, before its derived columns and hoisted
+       constants are computed; the batch's first evaluation is timed from it
+       when prep_timed is set. */
+    hipEvent_t tp = nullptr;
+    int prep_timed = 0;
     hipEvent_t ev_done = nullptr;
 
     /* Pending evaluator batch (submit/collect split). */
@@ -128,7 +140,7 @@ void rc_fit_state_free(RcCtx *c);
    (rc_eval.hip).  Returns the batch's peak stack depth. */
 int rc_translate_programs(const uint32_t *prog, const uint32_t *off, int nprog, int nvars,
                           std::vector<uint32_t> &code, std::vector<uint32_t> &meta,
-                          int cls_start[5]);
+                          int cls_start[5], std::vector<uint32_t> *derived, int max_derived);
 
 #define RC_CHECK(x) do { if ((x) != hipSuccess) return -1; } while (0)
 

@@ -93,7 +93,7 @@ SIGNATURES = {
     "rch_fitvm_compile": (C.c_int, [u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, u32p,
                                     C.c_int, u32p, C.c_int, i32p]),
     "rch_eval_translate": (C.c_int, [u32p, u32p, C.c_int, C.c_int, u32p, C.c_int, u32p, C.c_int,
-                                     i32p]),
+                                     u32p, C.c_int, i32p, i32p]),
 }
 
 _lib = None
