@@ -469,8 +469,8 @@ __device__ __forceinline__ void rc_fused(uint32_t bop, float *tos, Leaf leaf)
     }
 }
 
-/* Evaluate the program at `pc` for R rows of this lane.  `xrow[r]` is the
-   row index (clamped in range by the caller). */
+/* Evaluate the program at `pc` for R rows of this lane: rows row0 +
+   loff[r] / 4 (clamped in range by the caller). */
 /* Column c of the evaluator's rows: an input (c < nv) or a derived column. */
 __device__ __forceinline__ const float *rc_col(const float *cols, const float *dcols, int n, int nv,
                                                uint32_t c)
@@ -478,11 +478,19 @@ __device__ __forceinline__ const float *rc_col(const float *cols, const float *d
     return (c < (uint32_t)nv) ? cols + (size_t)c * (size_t)n : dcols + (size_t)(c - nv) * (size_t)n;
 }
 
+/* Row (row0 + loff / 4) of a column: a wave-uniform 64-bit base plus a
+   32-bit per-lane byte offset within the row tile. */
+__device__ __forceinline__ float rc_ld(const float *col, int row0, uint32_t loff)
+{
+    const char *b = (const char *)(col + row0);
+    return *(const float *)(b + loff);
+}
+
 template <int S, int R>
 __device__ __forceinline__ void rc_run(const uint32_t *__restrict__ pc,
                                        const float *__restrict__ cols,
                                        const float *__restrict__ dcols, int n, int nv,
-                                       const int *xrow, float *res)
+                                       int row0, const uint32_t *loff, float *res)
 {
     /* st[r][d]: the compiler keeps this in VGPRs (indexed by the uniform
        depth) as long as R * S <= 32; the per-class (S, R) choice keeps it so. */
@@ -505,7 +513,7 @@ __device__ __forceinline__ void rc_run(const uint32_t *__restrict__ pc,
             if (op == OP_VAR) {
                 const float *col = rc_col(cols, dcols, n, nv, w >> 16);
 #pragma unroll
-                for (int r = 0; r < R; ++r) tos[r] = col[xrow[r]];
+                for (int r = 0; r < R; ++r) tos[r] = rc_ld(col, row0, loff[r]);
             } else {
                 const float c = __uint_as_float(*pc++);
 #pragma unroll
@@ -518,7 +526,7 @@ __device__ __forceinline__ void rc_run(const uint32_t *__restrict__ pc,
                 const float *col = rc_col(cols, dcols, n, nv, w >> 16);
                 float lv[R];
 #pragma unroll
-                for (int r = 0; r < R; ++r) lv[r] = col[xrow[r]];
+                for (int r = 0; r < R; ++r) lv[r] = rc_ld(col, row0, loff[r]);
                 auto leaf = [&](int r) { return lv[r]; };
                 if (f & 2u) rc_fused<R, true>(bop, tos, leaf);
                 else rc_fused<R, false>(bop, tos, leaf);
@@ -630,22 +638,27 @@ __global__ __launch_bounds__(64) void rc_eval_kernel(
     double acc = 0.0;
     int bad = 0;
     for (int base = row0; base < row1; base += 64 * R) {
-        int xrow[R];
+        uint32_t loff[R];
         bool ok[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int row = base + r * 64 + lane;
             ok[r] = row < row1;
-            xrow[r] = ok[r] ? row : row0;
+            loff[r] = ok[r] ? (uint32_t)(row - row0) * 4u : 0u;
+        }
+        float yv[R];
+        if (MODE != MODE_PREDICT) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) yv[r] = rc_ld(y, row0, loff[r]);
         }
         float res[R];
-        rc_run<S, R>(pc, cols, dcols, n, nv, xrow, res);
+        rc_run<S, R>(pc, cols, dcols, n, nv, row0, loff, res);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (!ok[r]) continue;
             const float pred = res[r];
             if (MODE == MODE_PREDICT) {
-                pred_out[(size_t)p * (size_t)n + (size_t)xrow[r]] = pred;
+                pred_out[(size_t)p * (size_t)n + (size_t)(row0 + (loff[r] >> 2))] = pred;
                 continue;
             }
             const bool hard = !isfinite(pred) || fabsf(pred) > 1e30f;
@@ -654,7 +667,7 @@ __global__ __launch_bounds__(64) void rc_eval_kernel(
                     acc += 1e30;
                 } else {
                     /* rc_row_loss (eval.cu:648-664) on the f32 residual. */
-                    const float rr = pred - y[xrow[r]];
+                    const float rr = pred - yv[r];
                     float l;
                     if (kind == RC_LOSS_LOG_SQUASHED) {
                         const float q = rr * rr;
@@ -673,7 +686,7 @@ __global__ __launch_bounds__(64) void rc_eval_kernel(
                     bad = 1;
                     continue;
                 }
-                const double rr = (double)pred - (double)y[xrow[r]];
+                const double rr = (double)pred - (double)yv[r];
                 if (kind == RC_SCORE_ABSOLUTE) {
                     acc += fabs(rr);
                 } else if (kind == RC_SCORE_LOG_SQUASHED) {
@@ -713,9 +726,9 @@ __global__ __launch_bounds__(64) void rc_hoist_kernel(uint32_t *__restrict__ cod
 {
     const int h = blockIdx.x;
     if (h >= nhoist) return;
-    int xrow[1] = {0};
+    const uint32_t loff[1] = {0u};
     float res[1];
-    rc_run<RC_MAX_STACK, 1>(code + hoist[2 * h], nullptr, nullptr, 0, 0, xrow, res);
+    rc_run<RC_MAX_STACK, 1>(code + hoist[2 * h], nullptr, nullptr, 0, 0, 0, loff, res);
     if (threadIdx.x == 0) code[hoist[2 * h + 1]] = __float_as_uint(res[0]);
 }
 
