@@ -278,3 +278,123 @@ def test_large_batch_row_tiling_is_exact_at_tile_edges():
         r = p.astype(np.float64) - y
         assert v == pytest.approx(np.sum(r * r) / n, rel=1e-12)
     ev.close()
+
+
+def _f64_pred_rows(pred):
+    p = pred.astype(np.float64)
+    hard = ~np.isfinite(p) | (np.abs(p) > 1e30)
+    return p, hard
+
+
+@pytest.mark.parametrize("kind", [5, 6])
+def test_correlation_scores_match_an_f64_reducer(data, kind):
+    """1 - rho (kind 6) or 1 - |rho| (kind 5) of the device's f32 predictions
+    against the f32 target, centred on the prediction mean and the target's
+    cached mean (eval.cu:930-1010); a non-finite row is status 2, zero
+    variance is rho = 0."""
+    x, y, ev = data
+    wl = workloads.config1()
+    trees = wl.trees[:48] + [node(ADD, const(2.0), const(1.0)), node(LOG, node(NEG, node(
+        MUL, var(0), var(0))))]
+    pop = host.Population()
+    for t in trees:
+        pop.add(t)
+    prog, off = pop.arrays()
+    pred = ev.predict_words(prog, off)
+    vals, st, ms = ev.score_words(prog, off, kind)
+    assert ms >= 0
+    y32 = y.astype(np.float32).astype(np.float64)
+    ybar = y32.mean()
+    syy = float(np.sum((y32 - ybar) ** 2))
+    checked = 0
+    for i in range(len(trees)):
+        p, hard = _f64_pred_rows(pred[i])
+        if hard.any():
+            assert st[i] == 2 and vals[i] == math.inf, i
+            continue
+        assert st[i] == 0, i
+        if np.ptp(p) == 0.0:
+            # constant prediction: the centred sums are rounding noise
+            assert 0.0 <= vals[i] <= 2.0
+            continue
+        pbar = p.mean()
+        spp = float(np.sum((p - pbar) ** 2))
+        spy = float(np.sum((p - pbar) * (y32 - ybar)))
+        rho = 0.0 if spp <= 0 or syy <= 0 else spy / (math.sqrt(spp) * math.sqrt(syy))
+        want = 1.0 - rho if kind == 6 else 1.0 - abs(rho)
+        assert vals[i] == pytest.approx(want, rel=1e-9, abs=1e-12), i
+        checked += 1
+    assert checked > 30 and st[-1] == 2
+
+
+def _class_loss_f32(p, t, kind):
+    """The classification row losses in f32 (eval.cu:725-751)."""
+    p = p.astype(np.float32)
+    t = t.astype(np.float32)
+    one = np.float32(1.0)
+    with np.errstate(all="ignore"):
+        u = t * p
+        if kind == 10:
+            a = np.abs(u)
+            return np.where(u < 0, -u, np.float32(0)) + np.log1p(np.exp(-a))
+        sig = np.where(p >= 0, one / (one + np.exp(-p)), np.exp(p) / (one + np.exp(p)))
+        sig = sig.astype(np.float32)
+        if kind == 11:
+            d = sig - np.float32(0.5) * (t + one)
+            return d * d
+        if kind == 12:
+            h = one - t * (np.float32(2.0) * sig - one)
+            return np.maximum(h, np.float32(0))
+        h = one - u
+        return np.maximum(h, np.float32(0)) if kind == 13 else np.where(h > 0, h * h, np.float32(0))
+
+
+@pytest.mark.parametrize("kind", [10, 11, 12, 13, 14])
+def test_classification_scores_match_an_f64_reducer(data, kind):
+    """Weighted mean of the f32 row losses of logit predictions against label
+    signs (eval.cu:884-926): zero-weight rows are ignored, a non-finite
+    prediction at a weighted row is status 2, no positive weight is status 4,
+    and a context without label signs refuses the launch."""
+    x, y, _ = data
+    n, v = x.shape
+    cols = x.T.astype(np.float32).reshape(-1)
+    ev = host.GpuEvaluator(cols, n, v, y.astype(np.float32))
+    wl = workloads.config1()
+    trees = wl.trees[:40]
+    pop = host.Population()
+    for t in trees:
+        pop.add(t)
+    prog, off = pop.arrays()
+    _, _, ms = ev.score_words(prog, off, kind)
+    assert ms < 0  # no label signs yet
+    rng = np.random.default_rng(kind)
+    sign = np.where(y > 0.1, 1.0, -1.0)
+    w = rng.uniform(0.5, 2.0, n)
+    w[rng.random(n) < 0.2] = 0.0
+    assert ev.set_fit_sign(sign) and ev.set_score_weights(w)
+    pred = ev.predict_words(prog, off)
+    vals, st, ms = ev.score_words(prog, off, kind)
+    assert ms >= 0
+    tol = 1e-12 if kind in (13, 14) else 2e-6
+    checked = 0
+    for i in range(len(trees)):
+        p, hard = _f64_pred_rows(pred[i])
+        live = w > 0
+        if (hard & live).any():
+            assert st[i] == 2 and vals[i] == math.inf, i
+            continue
+        loss = _class_loss_f32(pred[i][live], sign[live], kind).astype(np.float64)
+        want = float(np.sum(w[live] * loss) / np.sum(w[live]))
+        assert st[i] == 0 and vals[i] == pytest.approx(want, rel=tol, abs=1e-12), (i, kind)
+        checked += 1
+    assert checked > 20
+    # all weights zero: status 4; a negative weight: status 2
+    assert ev.set_score_weights(np.zeros(n))
+    vals, st, _ = ev.score_words(prog[:int(off[1])], off[:2], kind)
+    assert st[0] == 4 and vals[0] == math.inf
+    bad = w.copy()
+    bad[7] = -1.0
+    assert ev.set_score_weights(bad)
+    vals, st, _ = ev.score_words(prog[:int(off[1])], off[:2], kind)
+    assert st[0] == 2
+    ev.close()

@@ -185,6 +185,8 @@ void rc_destroy(void *ctx)
     }
     if (c->d_part) (void)hipFree(c->d_part);
     if (c->d_part_bad) (void)hipFree(c->d_part_bad);
+    if (c->d_part2) (void)hipFree(c->d_part2);
+    if (c->d_pbar) (void)hipFree(c->d_pbar);
     if (c->d_out) (void)hipFree(c->d_out);
     if (c->d_score) (void)hipFree(c->d_score);
     if (c->d_status) (void)hipFree(c->d_status);

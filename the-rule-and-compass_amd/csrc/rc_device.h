@@ -56,6 +56,15 @@ enum {
 #define RC_SCORE_AICC 8
 #define RC_SCORE_BIC 9
 #define RC_SCORE_NONFINITE 2
+/* Classification score codes (eval.cu:628-640): the prediction is a logit p,
+   the label sign t = 2l - 1 comes from rc_set_fit_sign, rows are weighted by
+   the score weights and the score is the weighted mean row loss. */
+#define RC_SCORE_LOG_LOSS 10
+#define RC_SCORE_BRIER 11
+#define RC_SCORE_HINGE_SIGMOID 12
+#define RC_SCORE_HINGE_ORDINARY 13
+#define RC_SCORE_HINGE_SQUARED 14
+#define RC_SCORE_ALL_ZERO_WEIGHT 4
 
 #define RC_F32_NAN __builtin_nanf("")
 #define RC_F64_NAN __builtin_nan("")
@@ -63,6 +72,47 @@ enum {
 /* ---- f32 row semantics (the evaluator and the mixed-precision fitter). */
 
 __device__ __forceinline__ float rc_logistic_f(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+/* Logistic without overflow for either sign of x (the two-branch form
+   eval.cu:707-712 uses for classification scores). */
+__device__ __forceinline__ float rc_logistic_stable_f(float x)
+{
+    if (x >= 0.0f) return 1.0f / (1.0f + expf(-x));
+    const float e = expf(x);
+    return e / (1.0f + e);
+}
+
+/* Classification row loss of logit p against label sign t (eval.cu:725-751):
+   log loss log(1 + e^{-tp}) in its stable form, Brier (sigma(p) - l)^2 with
+   l = (t + 1) / 2, hinge on the sigmoid margin, ordinary and squared hinge.
+   Any other code is NaN. */
+__device__ __forceinline__ float rc_class_row_loss_f32(float p, float t, int kind)
+{
+    const float u = t * p;
+    switch (kind) {
+        case RC_SCORE_LOG_LOSS: {
+            const float a = fabsf(u);
+            return ((u < 0.0f) ? -u : 0.0f) + log1pf(expf(-a));
+        }
+        case RC_SCORE_BRIER: {
+            const float d = rc_logistic_stable_f(p) - 0.5f * (t + 1.0f);
+            return d * d;
+        }
+        case RC_SCORE_HINGE_SIGMOID: {
+            const float h = 1.0f - t * (2.0f * rc_logistic_stable_f(p) - 1.0f);
+            return h > 0.0f ? h : 0.0f;
+        }
+        case RC_SCORE_HINGE_ORDINARY: {
+            const float h = 1.0f - u;
+            return h > 0.0f ? h : 0.0f;
+        }
+        case RC_SCORE_HINGE_SQUARED: {
+            const float h = 1.0f - u;
+            return h > 0.0f ? h * h : 0.0f;
+        }
+        default: return RC_F32_NAN;
+    }
+}
 
 /* Unary operator on f32, the reference device semantics (eval.cu:519-563). */
 __device__ __forceinline__ float rc_unary_f32(unsigned op, float a)

@@ -57,6 +57,10 @@ constexpr uint32_t RC_OP_NAN = 254;
    input column (w >> 16) (var = 1) or the f32 word that follows (var = 0). */
 constexpr uint32_t RC_OP_FUSED = 64;
 
+/* Internal score pass: the per-program mean prediction that the correlation
+   scores centre on (their first pass, eval.cu:930-960). */
+constexpr int RC_SCORE_CORR_MEAN = 100;
+
 constexpr int MODE_EVAL = 0;    /* sum of rho with 1e30 hard failures (rc_eval) */
 constexpr int MODE_SCORE = 1;   /* f64 score objectives + bad flag */
 constexpr int MODE_PREDICT = 2; /* raw predictions */
@@ -471,6 +475,16 @@ __device__ __forceinline__ void rc_fused(uint32_t bop, float *tos, Leaf leaf)
 
 /* Evaluate the program at `pc` for R rows of this lane: rows row0 +
    loff[r] / 4 (clamped in range by the caller). */
+/* Score inputs beyond the rows: score weights (may be null), label signs,
+   per-program mean predictions, the target mean, second per-item sums. */
+struct RcScoreArgs {
+    const double *w;
+    const float *tsign;
+    const double *pbar;
+    double ybar;
+    double *part2;
+};
+
 /* Column c of the evaluator's rows: an input (c < nv) or a derived column. */
 __device__ __forceinline__ const float *rc_col(const float *cols, const float *dcols, int n, int nv,
                                                uint32_t c)
@@ -626,7 +640,8 @@ __global__ __launch_bounds__(64) void rc_eval_kernel(
     const float *__restrict__ y, int n,
     const uint32_t *__restrict__ code, const uint32_t *__restrict__ meta,
     const uint32_t *__restrict__ plist, int ncls, int tiles, int tile_rows, int kind,
-    double *__restrict__ part, int *__restrict__ part_bad, float *__restrict__ pred_out)
+    double *__restrict__ part, int *__restrict__ part_bad, float *__restrict__ pred_out,
+    RcScoreArgs sa)
 {
     int j, t;
     if (!rc_item_of(blockIdx.x, ncls, tiles, &j, &t)) return;
@@ -635,8 +650,13 @@ __global__ __launch_bounds__(64) void rc_eval_kernel(
     const uint32_t *pc = code + meta[p];
     const int row0 = t * tile_rows;
     const int row1 = min(n, row0 + tile_rows);
-    double acc = 0.0;
+    double acc = 0.0, acc2 = 0.0;
     int bad = 0;
+    const bool corr = MODE == MODE_SCORE &&
+                      (kind == RC_SCORE_CORRELATION || kind == RC_SCORE_CORRELATION_SIGNED);
+    const bool classif = MODE == MODE_SCORE && kind >= RC_SCORE_LOG_LOSS &&
+                         kind <= RC_SCORE_HINGE_SQUARED;
+    const double pb = corr ? sa.pbar[p] : 0.0;
     for (int base = row0; base < row1; base += 64 * R) {
         uint32_t loff[R];
         bool ok[R];
@@ -682,8 +702,38 @@ __global__ __launch_bounds__(64) void rc_eval_kernel(
                     acc += (double)l;
                 }
             } else {
+                if (classif) {
+                    /* eval.cu:884-901: invalid weight -> bad; zero weight ->
+                       row ignored; then label and prediction must be finite. */
+                    const int row = row0 + (int)(loff[r] >> 2);
+                    const double wi = sa.w ? sa.w[row] : 1.0;
+                    if (!isfinite(wi) || wi < 0.0) {
+                        bad = 1;
+                        continue;
+                    }
+                    if (wi == 0.0) continue;
+                    const float tt = sa.tsign[row];
+                    if (!isfinite(tt) || hard) {
+                        bad = 1;
+                        continue;
+                    }
+                    acc += wi * (double)rc_class_row_loss_f32(pred, tt, kind);
+                    acc2 += wi;
+                    continue;
+                }
                 if (hard) {
                     bad = 1;
+                    continue;
+                }
+                if (kind == RC_SCORE_CORR_MEAN) {
+                    acc += (double)pred;
+                    continue;
+                }
+                if (corr) {
+                    const double dp = (double)pred - pb;
+                    const double dy = (double)yv[r] - sa.ybar;
+                    acc += dp * dp;
+                    acc2 += dp * dy;
                     continue;
                 }
                 const double rr = (double)pred - (double)yv[r];
@@ -711,10 +761,13 @@ __global__ __launch_bounds__(64) void rc_eval_kernel(
         tot = rc_wave_sum_f64(acc);
     }
     const int anybad = __any(bad) ? 1 : 0;
+    const bool two = corr || classif;
+    const double tot2 = two ? rc_wave_sum_f64(acc2) : 0.0;
     if (lane == 0) {
         const size_t item = (size_t)p * (size_t)tiles + (size_t)t;
         part[item] = tot;
         part_bad[item] = anybad;
+        if (two) sa.part2[item] = tot2;
     }
 }
 
@@ -747,18 +800,23 @@ __global__ __launch_bounds__(256) void rc_derive_kernel(const float *__restrict_
 
 /* Per-program finish in tile order (deterministic). */
 __global__ void rc_eval_finish(const double *__restrict__ part, const int *__restrict__ part_bad,
-                               int nprog, int tiles, int n, int mode, int kind,
-                               float *__restrict__ out32, double *__restrict__ out64,
-                               int *__restrict__ status)
+                               const double *__restrict__ part2, int nprog, int tiles, int n,
+                               int mode, int kind, double syy, float *__restrict__ out32,
+                               double *__restrict__ out64, int *__restrict__ status,
+                               double *__restrict__ pbar)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nprog) return;
-    double s = 0.0;
+    const bool two = mode == MODE_SCORE &&
+                     (kind == RC_SCORE_CORRELATION || kind == RC_SCORE_CORRELATION_SIGNED ||
+                      (kind >= RC_SCORE_LOG_LOSS && kind <= RC_SCORE_HINGE_SQUARED));
+    double s = 0.0, s2 = 0.0;
     int bad = 0;
     for (int t = 0; t < tiles; ++t) {
         const size_t item = (size_t)p * (size_t)tiles + (size_t)t;
         if (mode == MODE_SCORE && kind == RC_SCORE_MAXIMUM) s = fmax(s, part[item]);
         else s += part[item];
+        if (two) s2 += part2[item];
         bad |= part_bad[item];
     }
     if (mode == MODE_EVAL) {
@@ -771,10 +829,30 @@ __global__ void rc_eval_finish(const double *__restrict__ part, const int *__res
         return;
     }
     status[p] = 0;
-    if (kind == RC_SCORE_AIC || kind == RC_SCORE_AICC || kind == RC_SCORE_BIC) out64[p] = s;
-    else if (kind == RC_SCORE_SIGNED) out64[p] = fabs(s) / (double)n;
-    else if (kind == RC_SCORE_MAXIMUM) out64[p] = s;
-    else out64[p] = s / (double)n;
+    if (kind == RC_SCORE_CORR_MEAN) {
+        pbar[p] = s / (double)n;
+    } else if (kind >= RC_SCORE_LOG_LOSS && kind <= RC_SCORE_HINGE_SQUARED) {
+        /* Weighted mean; no positive weight is status 4 (eval.cu:917-925). */
+        if (!(s2 > 0.0)) {
+            out64[p] = __builtin_inf();
+            status[p] = RC_SCORE_ALL_ZERO_WEIGHT;
+        } else {
+            out64[p] = s / s2;
+        }
+    } else if (kind == RC_SCORE_CORRELATION || kind == RC_SCORE_CORRELATION_SIGNED) {
+        /* 1 - rho (signed) or 1 - |rho|; zero variance on either side is
+           rho = 0 (eval.cu:1000-1008). */
+        const double rho = (s <= 0.0 || syy <= 0.0) ? 0.0 : s2 / (sqrt(s) * sqrt(syy));
+        out64[p] = kind == RC_SCORE_CORRELATION_SIGNED ? 1.0 - rho : 1.0 - fabs(rho);
+    } else if (kind == RC_SCORE_AIC || kind == RC_SCORE_AICC || kind == RC_SCORE_BIC) {
+        out64[p] = s;
+    } else if (kind == RC_SCORE_SIGNED) {
+        out64[p] = fabs(s) / (double)n;
+    } else if (kind == RC_SCORE_MAXIMUM) {
+        out64[p] = s;
+    } else {
+        out64[p] = s / (double)n;
+    }
 }
 
 /* FNV-1a over a fixed-width slot up to and including HALT (eval.cu:2665-2681). */
@@ -815,18 +893,19 @@ void launch_class(int mode, dim3 grid, hipStream_t s, const RcCtx *c, const uint
                   int kind, float *pred)
 {
     const RcRows *r = &c->sets[0];
+    const RcScoreArgs sa = {r->w64, c->d_tsign, c->d_pbar, r->ybar, c->d_part2};
     if (mode == MODE_EVAL)
         rc_eval_kernel<S, R, MODE_EVAL><<<grid, 64, 0, s>>>(
             r->cols, c->d_derived, r->v, r->y, r->n, code, meta, plist, ncls, tiles, tile_rows, kind,
-            c->d_part, c->d_part_bad, pred);
+            c->d_part, c->d_part_bad, pred, sa);
     else if (mode == MODE_SCORE)
         rc_eval_kernel<S, R, MODE_SCORE><<<grid, 64, 0, s>>>(
             r->cols, c->d_derived, r->v, r->y, r->n, code, meta, plist, ncls, tiles, tile_rows, kind,
-            c->d_part, c->d_part_bad, pred);
+            c->d_part, c->d_part_bad, pred, sa);
     else
         rc_eval_kernel<S, R, MODE_PREDICT><<<grid, 64, 0, s>>>(
             r->cols, c->d_derived, r->v, r->y, r->n, code, meta, plist, ncls, tiles, tile_rows, kind,
-            c->d_part, c->d_part_bad, pred);
+            c->d_part, c->d_part_bad, pred, sa);
 }
 
 int launch_rows(RcCtx *c, int mode, int kind, int nprog, const uint32_t *code,
@@ -840,6 +919,10 @@ int launch_rows(RcCtx *c, int mode, int kind, int nprog, const uint32_t *code,
     const size_t items = (size_t)nprog * (size_t)tiles;
     if (rc_grow(c, (void **)&c->d_part, &c->part_cap, items, sizeof(double)) != 0) return -1;
     if (rc_grow(c, (void **)&c->d_part_bad, &c->part_bad_cap, items, sizeof(int)) != 0) return -1;
+    if (mode == MODE_SCORE &&
+        (rc_grow(c, (void **)&c->d_part2, &c->part2_cap, items, sizeof(double)) != 0 ||
+         rc_grow(c, (void **)&c->d_pbar, &c->pbar_cap, (size_t)nprog, sizeof(double)) != 0))
+        return -1;
     for (int cls = 0; cls < 4; ++cls) {
         const int ncls = c->cls_start[cls + 1] - c->cls_start[cls];
         if (ncls <= 0) continue;
@@ -864,9 +947,9 @@ int launch_rows(RcCtx *c, int mode, int kind, int nprog, const uint32_t *code,
         if (hipGetLastError() != hipSuccess) return -1;
     }
     if (mode == MODE_PREDICT) return 0;
-    rc_eval_finish<<<(nprog + 127) / 128, 128, 0, s>>>(c->d_part, c->d_part_bad, nprog, tiles, n,
-                                                       mode, kind, c->d_out, c->d_score,
-                                                       c->d_status);
+    rc_eval_finish<<<(nprog + 127) / 128, 128, 0, s>>>(c->d_part, c->d_part_bad, c->d_part2, nprog,
+                                                       tiles, n, mode, kind, r->syy, c->d_out,
+                                                       c->d_score, c->d_status, c->d_pbar);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1019,14 +1102,19 @@ int ensure_outputs(RcCtx *c, int nprog)
 float score_current(RcCtx *c, int nprog, int kind, const int *pcount, double *out, int *status)
 {
     RcStats *st = &c->stats;
-    if (!(kind >= 0 && kind <= 4) && !(kind >= 7 && kind <= 9)) {
+    const bool classif = kind >= RC_SCORE_LOG_LOSS && kind <= RC_SCORE_HINGE_SQUARED;
+    /* A classification score needs the label signs (eval.cu:3664-3670). */
+    if (!(kind >= 0 && kind <= 9) && !(classif && c->has_tsign)) {
         st->failed = 1;
         return -1.0f;
     }
     if (ensure_outputs(c, nprog) != 0) return -1.0f;
     double t = rc_now_ms();
     (void)hipEventRecord(c->t0, c->s_eval);
-    if (launch_rows(c, MODE_SCORE, kind, nprog, c->arena_code[c->arena_slot],
+    const bool corr = kind == RC_SCORE_CORRELATION || kind == RC_SCORE_CORRELATION_SIGNED;
+    if ((corr && launch_rows(c, MODE_SCORE, RC_SCORE_CORR_MEAN, nprog, c->arena_code[c->arena_slot],
+                             c->arena_meta[c->arena_slot], c->s_eval, nullptr) != 0) ||
+        launch_rows(c, MODE_SCORE, kind, nprog, c->arena_code[c->arena_slot],
                     c->arena_meta[c->arena_slot], c->s_eval, nullptr) != 0) {
         st->failed = 1;
         return -1.0f;

@@ -76,6 +76,10 @@ struct RcCtx {
     size_t part_cap = 0;
     int *d_part_bad = nullptr;
     size_t part_bad_cap = 0;
+    double *d_part2 = nullptr; /* second per-item sum (weights, cross products) */
+    size_t part2_cap = 0;
+    double *d_pbar = nullptr;  /* per-program mean prediction (correlation scores) */
+    size_t pbar_cap = 0;
     float *d_out = nullptr;
     size_t out_cap = 0;
     double *d_score = nullptr;

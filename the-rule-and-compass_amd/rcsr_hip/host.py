@@ -219,6 +219,16 @@ class GpuEvaluator:
                                    _L.ptr(y, _L.f32p), _L.ptr(c64, _L.f64p), _L.ptr(yy, _L.f64p),
                                    _L.ptr(ww, _L.f64p))
 
+    def set_score_weights(self, w=None) -> bool:
+        """Per-row weights of the classification scores (None clears them)."""
+        ww = None if w is None else np.ascontiguousarray(w, np.float64)
+        return self._L.rc_set_score_weights(self.ctx, _L.ptr(ww, _L.f64p)) == 0
+
+    def set_fit_sign(self, sign=None) -> bool:
+        """Label signs t = 2l - 1 of the classification scores (None clears)."""
+        ss = None if sign is None else np.ascontiguousarray(sign, np.float64)
+        return self._L.rc_set_fit_sign(self.ctx, _L.ptr(ss, _L.f64p)) == 0
+
     def set_tuning(self, key: int, value: int) -> int:
         """Set a launch shape (TUNE_*; 0 = the built-in choice); returns the
         previous value, -1 when the key or value is refused."""
