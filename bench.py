@@ -241,10 +241,18 @@ def main():
         rel = np.abs(mixed_loss[:ns][ok] - cl[ok]) / cl[ok]
         log(f"cpu oracle: {ns} candidates on {threads} threads in {tc:.1f}s; mixed vs oracle "
             f"median {np.median(rel):.2e} max {np.max(rel):.2e}")
+        # The reference's CPU fallback runs min(cores, 8) workers.
+        w8 = min(os.cpu_count() or 1, 8)
+        ts = time.perf_counter()
+        oracle.fit_batch(wl.trees[:ns], wl.x, wl.y, None, args.iters, threads=w8, kmax=20)
+        t8 = time.perf_counter() - ts
+        log(f"cpu oracle at {w8} workers: {ns / t8:.2f} candidates/s")
         result["cpu_baseline"] = {
             "value": ns / tc, "unit": "candidates/s", "cores": threads, "kind": "port",
             "sample": f"first {ns} of the {len(plans)} candidates, full N={wl.n}, "
-                      f"{args.iters} LM iterations + 4 polish, one candidate per thread"}
+                      f"{args.iters} LM iterations + 4 polish, one candidate per thread",
+            "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
+            "value_at_min_cores_8": ns / t8, "workers_at_min_cores_8": w8}
         result["loss_rel_err_vs_cpu_oracle"] = {
             "median": float(np.median(rel)), "max": float(np.max(rel)),
             "compared": int(ok.sum())}
@@ -331,6 +339,17 @@ def tree_vars(t):
             seen.add(int(n.val))
         stack.extend(n.kids)
     return seen
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def eval_part(args, host, workloads):
