@@ -64,6 +64,15 @@ typedef struct RcArenaReport {
     unsigned long long fs_pinned_bytes;
 } RcArenaReport;
 
+/* eval.cu:4529-4535 RcMaxOut: one segment of rc_max_scan_batch. */
+typedef struct RcMaxOut {
+    double vmax; /* exact max |r|, NaN when status != 0 */
+    double lo;   /* max of the per-row lower bounds (a level was requested) */
+    double hi;   /* min of the per-row upper bounds */
+    int row;     /* lowest-index maximiser, -1 when status != 0 */
+    int status;  /* 0 ok; 2 a row was not finite */
+} RcMaxOut;
+
 /* ---- context and data -------------------------------------------------- */
 
 /* eval.cu:3137 rc_create. Uploads n x v f32 columns and the f32 target. */
@@ -139,6 +148,45 @@ float rc_fit_batch(void *ctx, const unsigned int *prog, const unsigned int *off,
    whose largest constant count is kmax (threads per workgroup, LDS bytes per
    workgroup, constant cap); -1 when kmax > 20. */
 int rc_fit_kernel_info(int kmax, int *threads, int *smem_bytes, int *max_constants);
+
+/* ---- exact order, rank and maximum statistics --------------------------- */
+
+/* Each call scores nsets x ntrials segments: segment set * ntrials + trial
+   evaluates set `set`'s post-order fit records (the rc_fit_batch program
+   form, pcount constants) over the resident f64 rows with the constants
+   c0[(set * ntrials + trial) * kmax ...].  kmax <= 20, at most 96 nodes,
+   nsets * ntrials * n < 2^31.  They return the device time in ms, negative
+   on failure (the caller then uses its exact CPU arm). */
+
+/* eval.cu:4743 rc_order_stat_score_batch: the median (what 0) or the
+   interquartile mean (what 1, ranks [n/4, 3n/4), all rows below four) of
+   |pred - y| per segment; status 2 when a row is not finite (value NaN). */
+float rc_order_stat_score_batch(void *ctx, const unsigned int *prog, const unsigned int *off,
+                                const int *pcount, int nsets, int ntrials, int kmax,
+                                const double *c0, int what, double *out_value, int *out_status);
+
+/* eval.cu:4270 rc_rank_score_batch: exact rank sums against the target.
+   With A and B the doubled centred average ranks (2 r - (n + 1), ties share
+   their average) of the prediction and of y: out[4 seg ..] = Sigma A B and
+   Sigma A A as two's-complement 128-bit (hi, lo) pairs, out_bb = Sigma B B;
+   status 2 when a prediction is not finite; n <= 1 or a non-finite target
+   fails the call. */
+float rc_rank_score_batch(void *ctx, const unsigned int *prog, const unsigned int *off,
+                          const int *pcount, int nsets, int ntrials, int kmax, const double *c0,
+                          unsigned long long *out, int *out_status, unsigned long long *out_bb);
+
+/* eval.cu:4852 rc_rank_debug_b: the target's cached B per row. */
+int rc_rank_debug_b(void *ctx, long long *out);
+
+/* eval.cu:4651 rc_max_scan_batch: per segment, the exact max |pred - y| and
+   its lowest row; with coords[seg] a constant slot (< pcount) and levels[seg]
+   a finite level t >= 0, also the interval [lo, hi] of steps delta along
+   that constant with |r_i + (d r_i / d c) delta| <= t on every row (a row
+   whose derivative is 0 and whose |r_i| > t makes it empty: lo = +inf,
+   hi = -inf).  coords / levels may be NULL (maximum only). */
+float rc_max_scan_batch(void *ctx, const unsigned int *prog, const unsigned int *off,
+                        const int *pcount, int nsets, int ntrials, int kmax, const double *c0,
+                        const int *coords, const double *levels, RcMaxOut *out);
 
 /* NEW: select the device arithmetic of rc_fit_batch for this context.
    0 = mixed precision (FP32 interpreter and adjoint, FP32-MFMA split Gram,

@@ -60,6 +60,19 @@ def lib():
         L.or_implicit_fit_batch.restype = None
         L.or_implicit_fit_batch.argtypes = [_ip, _dp, _ip, C.c_int, _dp, C.c_int, C.c_int, _dp,
                                             C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.or_predict_flat.restype = None
+        L.or_predict_flat.argtypes = [_ip, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp]
+        L.or_median_abs.restype = C.c_double
+        L.or_median_abs.argtypes = [_dp, C.c_int]
+        L.or_iqr_mean_abs.restype = C.c_double
+        L.or_iqr_mean_abs.argtypes = [_dp, C.c_int]
+        L.or_centred_ranks_doubled.restype = C.c_int
+        L.or_centred_ranks_doubled.argtypes = [_dp, C.c_int, C.POINTER(C.c_longlong)]
+        L.or_rank_sums.restype = None
+        L.or_rank_sums.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.c_int,
+                                   C.POINTER(C.c_ulonglong)]
+        L.or_max_scan.restype = None
+        L.or_max_scan.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, _ip, _dp, _dp, _ip]
         _lib = L
     return _lib
 
@@ -233,3 +246,58 @@ def implicit_fit_batch(trees: Sequence, X, w=None, iters=8, threads=1, kmax=8):
                                 _d(ww), int(iters), int(kmax), int(threads), _d(out_c),
                                 _d(out_l))
     return out_l, out_c
+
+
+# -- order, rank and maximum statistics -----------------------------------
+
+def median_abs(r) -> float:
+    r = np.ascontiguousarray(r, np.float64)
+    return float(lib().or_median_abs(_d(r), len(r)))
+
+
+def iqr_mean_abs(r) -> float:
+    r = np.ascontiguousarray(r, np.float64)
+    return float(lib().or_iqr_mean_abs(_d(r), len(r)))
+
+
+def centred_ranks_doubled(v) -> Optional[np.ndarray]:
+    v = np.ascontiguousarray(v, np.float64)
+    out = np.zeros(len(v), np.int64)
+    rc = lib().or_centred_ranks_doubled(_d(v), len(v), out.ctypes.data_as(C.POINTER(C.c_longlong)))
+    return None if rc != 0 else out
+
+
+def _i128(hi, lo) -> int:
+    u = (int(hi) << 64) | int(lo)
+    return u - (1 << 128) if u >> 127 else u
+
+
+def rank_sums(a, b) -> Tuple[int, int, int]:
+    """Exact (sum a*b, sum a*a, sum b*b) of two centred doubled rank vectors."""
+    a = np.ascontiguousarray(a, np.int64)
+    b = np.ascontiguousarray(b, np.int64)
+    out = np.zeros(6, np.uint64)
+    lp = C.POINTER(C.c_longlong)
+    lib().or_rank_sums(a.ctypes.data_as(lp), b.ctypes.data_as(lp), len(a),
+                       out.ctypes.data_as(C.POINTER(C.c_ulonglong)))
+    return _i128(out[0], out[1]), _i128(out[2], out[3]), _i128(out[4], out[5])
+
+
+def max_scan(r, b=None, level=float("nan")):
+    """(vmax, row, lo, hi, status) of the exact maximum-error scan."""
+    r = np.ascontiguousarray(r, np.float64)
+    bb = None if b is None else np.ascontiguousarray(b, np.float64)
+    vm, lo, hi = C.c_double(), C.c_double(), C.c_double()
+    row, st = C.c_int(), C.c_int()
+    lib().or_max_scan(_d(r), _d(bb), len(r), float(level), C.byref(vm), C.byref(row),
+                      C.byref(lo), C.byref(hi), C.byref(st))
+    return vm.value, row.value, lo.value, hi.value, st.value
+
+
+def predict_flat(tree, X) -> np.ndarray:
+    """Predictions of the flat (fit) form, the tree's own constants."""
+    ops, vals = _tree(tree)
+    X = _rows(X)
+    out = np.zeros(X.shape[0])
+    lib().or_predict_flat(_i(ops), _d(vals), len(ops), _d(X), X.shape[0], X.shape[1], _d(out))
+    return out

@@ -1077,6 +1077,24 @@ double or_sse(const int *ops, const double *vals, int nn, const double *x, int n
     return sse(f, D, c.data(), v);
 }
 
+void or_predict_flat(const int *ops, const double *vals, int nn, const double *x, int n, int nv,
+                     double *out)
+{
+    Node t;
+    if (!build_tree(ops, vals, nn, t)) {
+        for (int i = 0; i < n; ++i) out[i] = NaN;
+        return;
+    }
+    Flat f = flatten(t);
+    std::vector<double> c;
+    collect_consts(t, c);
+    std::vector<double> v(std::max<size_t>(f.op.size(), 1));
+    for (int i = 0; i < n; ++i) {
+        eval_row(f, x + (size_t)i * (size_t)nv, nv, c.data(), v);
+        out[i] = v[f.op.size() - 1];
+    }
+}
+
 void or_gradients(const int *ops, const double *vals, int nn, const double *x, int nv,
                   double *g, double *gx)
 {
@@ -1195,6 +1213,120 @@ void or_implicit_fit_batch(const int *ops, const double *vals, const int *tree_o
         collect_consts(tree, cc);
         for (size_t i = 0; i < cc.size() && (int)i < kmax; ++i) c[i] = cc[i];
     });
+}
+
+/* ---- order, rank and maximum statistics of residual vectors ----------- */
+
+/* metrics::median_abs (metrics.rs:1193-1205): the middle |r| of the sorted
+   absolute errors, 0.5 * (a + b) of the two middle values for even n. */
+double or_median_abs(const double *r, int n)
+{
+    if (n <= 0) return 0.0;
+    std::vector<double> s((size_t)n);
+    for (int i = 0; i < n; ++i) s[i] = std::fabs(r[i]);
+    std::sort(s.begin(), s.end());
+    return (n % 2 == 1) ? s[n / 2] : 0.5 * (s[n / 2 - 1] + s[n / 2]);
+}
+
+/* metrics::interquartile_mean_abs (metrics.rs:1209-1223): the mean, summed
+   left to right, of the sorted |r| at ranks [n/4, 3n/4) (all rows below
+   four). */
+double or_iqr_mean_abs(const double *r, int n)
+{
+    if (n <= 0) return 0.0;
+    std::vector<double> s((size_t)n);
+    for (int i = 0; i < n; ++i) s[i] = std::fabs(r[i]);
+    std::sort(s.begin(), s.end());
+    int lo = 0, hi = n;
+    if (n >= 4) {
+        lo = n / 4;
+        hi = std::max((3 * n) / 4, lo + 1);
+    }
+    double sum = 0.0;
+    for (int i = lo; i < hi; ++i) sum += s[i];
+    return sum / (double)(hi - lo);
+}
+
+/* metrics::centred_ranks_doubled (metrics.rs:1258-1286): A = 2 * average rank
+   - (n + 1) = i + j - n for the tie run at sorted positions [i, j); ties are
+   exact f64 equality (-0.0 == +0.0).  Returns -1 when a value is not
+   finite. */
+int or_centred_ranks_doubled(const double *v, int n, long long *out)
+{
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return -1;
+    std::vector<int> idx((size_t)n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return v[a] < v[b]; });
+    int i = 0;
+    while (i < n) {
+        int j = i + 1;
+        while (j < n && v[idx[j]] == v[idx[i]]) ++j;
+        const long long a = (long long)i + (long long)j - (long long)n;
+        for (int q = i; q < j; ++q) out[idx[q]] = a;
+        i = j;
+    }
+    return 0;
+}
+
+/* RankSums::between (metrics.rs:1307-1323): exact sum(a*b), sum(a*a),
+   sum(b*b) as two's-complement 128-bit (hi, lo) pairs in out[0..5]. */
+void or_rank_sums(const long long *a, const long long *b, int n, unsigned long long *out)
+{
+    __int128 ab = 0, aa = 0, bb = 0;
+    for (int i = 0; i < n; ++i) {
+        ab += (__int128)a[i] * (__int128)b[i];
+        aa += (__int128)a[i] * (__int128)a[i];
+        bb += (__int128)b[i] * (__int128)b[i];
+    }
+    const __int128 v[3] = {ab, aa, bb};
+    for (int k = 0; k < 3; ++k) {
+        const unsigned __int128 u = (unsigned __int128)v[k];
+        out[2 * k] = (unsigned long long)(u >> 64);
+        out[2 * k + 1] = (unsigned long long)u;
+    }
+}
+
+/* The exact maximum-error scan of one residual vector (the device stage
+   eval.cu:4505-4600): max |r| and its lowest row; with a finite level t >= 0
+   and the residuals' derivatives b along one constant, the intersection
+   [max lo_i, min hi_i] of {delta : |r_i + b_i delta| <= t}, where a row with
+   b_i == 0 only vetoes (|r_i| > t makes the level infeasible: lo = +inf,
+   hi = -inf).  A non-finite residual or derivative is status 2 (vmax NaN,
+   row -1).  b may be NULL (maximum only). */
+void or_max_scan(const double *r, const double *b, int n, double level, double *vmax, int *row,
+                 double *lo, double *hi, int *status)
+{
+    const bool want = b != nullptr && std::isfinite(level) && level >= 0.0;
+    double best = -1.0;
+    int brow = -1;
+    double L = -INF, H = INF;
+    bool bad = false, veto = false;
+    for (int i = 0; i < n; ++i) {
+        if (!std::isfinite(r[i]) || (want && !std::isfinite(b[i]))) {
+            bad = true;
+            break;
+        }
+        const double a = std::fabs(r[i]);
+        if (a > best) {
+            best = a;
+            brow = i;
+        }
+        if (!want) continue;
+        if (b[i] == 0.0) {
+            if (a > level) veto = true;
+        } else {
+            const double e1 = (-level - r[i]) / b[i];
+            const double e2 = (level - r[i]) / b[i];
+            L = std::max(L, std::fmin(e1, e2));
+            H = std::min(H, std::fmax(e1, e2));
+        }
+    }
+    *status = bad ? 2 : 0;
+    *vmax = bad ? NaN : (brow < 0 ? 0.0 : best);
+    *row = bad ? -1 : (brow < 0 ? 0 : brow);
+    *lo = veto ? INF : L;
+    *hi = veto ? -INF : H;
 }
 
 } // extern "C"

@@ -57,6 +57,11 @@ int or_nconst(const int *ops, const double *vals, int nn);
 double or_sse(const int *ops, const double *vals, int nn, const double *x, int n,
               int nv, const double *y, const double *w);
 
+/* Predictions of the flat (fit) form over n rows, the tree's own constants
+   (fit.rs:1424-1440 eval_row per row). */
+void or_predict_flat(const int *ops, const double *vals, int nn, const double *x, int n, int nv,
+                     double *out);
+
 /* Adjoint gradient of the root w.r.t. the constant slots (g has nconst
    entries) and w.r.t. the inputs (gx has nv entries); either may be NULL. */
 void or_gradients(const int *ops, const double *vals, int nn, const double *x,
@@ -107,6 +112,26 @@ void or_implicit_fit_batch(const int *ops, const double *vals,
                            const int *tree_off, int ntree, const double *x,
                            int n, int nv, const double *w, int iters, int kmax,
                            int threads, double *out_c, double *out_loss);
+
+/* Order, rank and maximum statistics of residual / prediction vectors. */
+
+/* metrics::median_abs (metrics.rs:1193-1205). */
+double or_median_abs(const double *r, int n);
+
+/* metrics::interquartile_mean_abs (metrics.rs:1209-1223). */
+double or_iqr_mean_abs(const double *r, int n);
+
+/* metrics::centred_ranks_doubled (metrics.rs:1258-1286); -1 if not finite. */
+int or_centred_ranks_doubled(const double *v, int n, long long *out);
+
+/* RankSums::between (metrics.rs:1307-1323): out = ab_hi, ab_lo, aa_hi,
+   aa_lo, bb_hi, bb_lo (two's-complement 128-bit). */
+void or_rank_sums(const long long *a, const long long *b, int n, unsigned long long *out);
+
+/* The exact maximum-error scan (eval.cu:4505-4600) of residuals r with
+   derivatives b along one constant (b may be NULL: maximum only). */
+void or_max_scan(const double *r, const double *b, int n, double level, double *vmax, int *row,
+                 double *lo, double *hi, int *status);
 
 #ifdef __cplusplus
 }

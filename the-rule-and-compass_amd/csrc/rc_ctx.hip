@@ -195,6 +195,7 @@ void rc_destroy(void *ctx)
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->h_out) (void)hipHostFree(c->h_out);
     rc_fit_state_free(c);
+    rc_order_free(c);
     if (c->s_eval) (void)hipStreamDestroy(c->s_eval);
     if (c->s_fit) (void)hipStreamDestroy(c->s_fit);
     if (c->t0) (void)hipEventDestroy(c->t0);
@@ -209,6 +210,7 @@ int rc_set_fit_data(void *ctx, const double *cols, const double *y, const double
     RcCtx *c = (RcCtx *)ctx;
     if (!c || !y || (!cols && c->sets[0].v > 0)) return -1;
     RcRows *r = &c->sets[0];
+    rc_order_invalidate(c);
     if (r->cols64) { (void)hipFree(r->cols64); r->cols64 = nullptr; }
     if (r->y64) { (void)hipFree(r->y64); r->y64 = nullptr; }
     if (r->w64) { (void)hipFree(r->w64); r->w64 = nullptr; }

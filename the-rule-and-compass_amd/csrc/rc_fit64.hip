@@ -41,39 +41,13 @@ struct P64Cand {
 __device__ __forceinline__ void p64_forward(const uint32_t *__restrict__ nd, int m,
                                             const FtRows &R, int row, const double *c, double *v)
 {
-    for (int i = 0; i < m; ++i) {
-        const uint32_t op = nd[4 * i], k0 = nd[4 * i + 1], k1 = nd[4 * i + 2], slot = nd[4 * i + 3];
-        if (op == OP_CONST) v[i] = c[slot];
-        else if (op == OP_VAR) v[i] = R.cols64[(size_t)slot * (size_t)R.n + (size_t)row];
-        else if (ft_is_binary(op)) v[i] = ft_binary(op, v[k0], v[k1]);
-        else v[i] = ft_unary(op, v[k0]);
-    }
+    ft_forward64(nd, m, R.cols64, R.n, row, c, v);
 }
 
 __device__ __forceinline__ void p64_backward(const uint32_t *__restrict__ nd, int m, const double *v,
                                              double *adj, double *g, int p)
 {
-    for (int i = 0; i < m; ++i) adj[i] = 0.0;
-    for (int j = 0; j < p; ++j) g[j] = 0.0;
-    adj[m - 1] = 1.0;
-    for (int i = m - 1; i >= 0; --i) {
-        const double a = adj[i];
-        if (a == 0.0) continue;
-        const uint32_t op = nd[4 * i], k0 = nd[4 * i + 1], k1 = nd[4 * i + 2], slot = nd[4 * i + 3];
-        if (op == OP_CONST) { g[slot] += a; continue; }
-        if (op == OP_VAR) continue;
-        const double x = v[k0];
-        if (op == OP_ADD) { adj[k0] += a; adj[k1] += a; }
-        else if (op == OP_SUB) { adj[k0] += a; adj[k1] -= a; }
-        else if (op == OP_MUL) { adj[k0] += a * v[k1]; adj[k1] += a * x; }
-        else if (op == OP_DIV) { const double y = v[k1]; adj[k0] += a / y; adj[k1] += -a * x / (y * y); }
-        else if (op == OP_POW) {
-            const double y = v[k1];
-            if (x > 0.0) { adj[k0] += a * y * pow(x, y - 1.0); adj[k1] += a * pow(x, y) * log(x); }
-        } else {
-            adj[k0] += a * ft_unary_partial<double>(op, x, v[i]);
-        }
-    }
+    ft_backward64(nd, m, v, adj, g, p);
 }
 
 __device__ __forceinline__ int p64_stat_kind(int s, int tri, int p, int *ei, int *ej)

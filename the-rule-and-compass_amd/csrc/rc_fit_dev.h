@@ -195,6 +195,52 @@ __device__ __forceinline__ T ft_unary_partial(unsigned op, T x, T out)
 /* ln(1e-12): the smallest identified fraction of parameter space. */
 #define FT_LN_DET_FLOOR (-27.631021115928547)
 
+/* ---- f64 row evaluation of the post-order records ------------------------ */
+
+/* Values of every node for one row in f64 (ft_forward_d, eval.cu:1324-1360):
+   v[i] is node i's value, v[m - 1] the prediction. */
+__device__ __forceinline__ void ft_forward64(const uint32_t *__restrict__ nd, int m,
+                                             const double *__restrict__ cols64, int n, int row,
+                                             const double *c, double *v)
+{
+    for (int i = 0; i < m; ++i) {
+        const uint32_t op = nd[4 * i], k0 = nd[4 * i + 1], k1 = nd[4 * i + 2], slot = nd[4 * i + 3];
+        if (op == OP_CONST) v[i] = c[slot];
+        else if (op == OP_VAR) v[i] = cols64[(size_t)slot * (size_t)n + (size_t)row];
+        else if (ft_is_binary(op)) v[i] = ft_binary(op, v[k0], v[k1]);
+        else v[i] = ft_unary(op, v[k0]);
+    }
+}
+
+/* Adjoint sweep of one row in f64 (ft_backward_d, eval.cu:1366-1463): g[j]
+   = d prediction / d constant j for the p constants; a node whose adjoint is
+   0 is skipped. */
+__device__ __forceinline__ void ft_backward64(const uint32_t *__restrict__ nd, int m, const double *v,
+                                              double *adj, double *g, int p)
+{
+    for (int i = 0; i < m; ++i) adj[i] = 0.0;
+    for (int j = 0; j < p; ++j) g[j] = 0.0;
+    adj[m - 1] = 1.0;
+    for (int i = m - 1; i >= 0; --i) {
+        const double a = adj[i];
+        if (a == 0.0) continue;
+        const uint32_t op = nd[4 * i], k0 = nd[4 * i + 1], k1 = nd[4 * i + 2], slot = nd[4 * i + 3];
+        if (op == OP_CONST) { g[slot] += a; continue; }
+        if (op == OP_VAR) continue;
+        const double x = v[k0];
+        if (op == OP_ADD) { adj[k0] += a; adj[k1] += a; }
+        else if (op == OP_SUB) { adj[k0] += a; adj[k1] -= a; }
+        else if (op == OP_MUL) { adj[k0] += a * v[k1]; adj[k1] += a * x; }
+        else if (op == OP_DIV) { const double y = v[k1]; adj[k0] += a / y; adj[k1] += -a * x / (y * y); }
+        else if (op == OP_POW) {
+            const double y = v[k1];
+            if (x > 0.0) { adj[k0] += a * y * pow(x, y - 1.0); adj[k1] += a * pow(x, y) * log(x); }
+        } else {
+            adj[k0] += a * ft_unary_partial<double>(op, x, v[i]);
+        }
+    }
+}
+
 /* Damped, Jacobi-equilibrated Cholesky solve of (A~ + lambda diag + 1e-12)
    delta~ = -D^-1/2 J^T r with the scale-free ln det gate on the undamped
    system (ft_solve, eval.cu:1874-1949).  jtj is the packed lower triangle

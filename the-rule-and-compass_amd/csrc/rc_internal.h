@@ -38,6 +38,36 @@ struct RcRows {
 
 struct RcFitState; /* rc_fit.hip */
 
+/* Buffers of the order / rank / maximum stages (rc_order.hip); device
+   pointers grow with rc_grow, the host vectors stage asynchronous uploads. */
+struct RcOrderBufs {
+    uint32_t *prog = nullptr;   size_t prog_cap = 0;
+    uint32_t *off = nullptr;    size_t off_cap = 0;
+    int *pcount = nullptr;      size_t pcount_cap = 0;
+    double *c0 = nullptr;       size_t c0_cap = 0;
+    int *seg = nullptr;         size_t seg_cap = 0;
+    unsigned long long *keys = nullptr;     size_t keys_cap = 0;
+    unsigned long long *keys_out = nullptr; size_t keys_out_cap = 0;
+    uint32_t *rows = nullptr;     size_t rows_cap = 0;
+    uint32_t *rows_out = nullptr; size_t rows_out_cap = 0;
+    int *start_of = nullptr;    size_t start_cap = 0;
+    void *temp = nullptr;       size_t temp_cap = 0;
+    int *status = nullptr;      size_t status_cap = 0;
+    double *val = nullptr;      size_t val_cap = 0;
+    unsigned long long *out = nullptr; size_t out_cap = 0;
+    long long *B = nullptr;     size_t B_cap = 0;    /* target's doubled centred ranks */
+    int *flag = nullptr;        size_t flag_cap = 0;
+    unsigned long long *bb = nullptr; size_t bb_cap = 0;
+    unsigned long long h_bb[2] = {0, 0};             /* Sigma B^2 (hi, lo) */
+    int rank_ready = 0;
+    RcMaxOut *mx = nullptr;     size_t mx_cap = 0;
+    int *coords = nullptr;      size_t coords_cap = 0;
+    double *levels = nullptr;   size_t levels_cap = 0;
+    std::vector<int> h_seg;
+    std::vector<int> h_coords;
+    std::vector<double> h_levels;
+};
+
 struct RcCtx {
     RcRows sets[RC_MAX_SETS];
     int nsets = 1;
@@ -68,6 +98,7 @@ struct RcCtx {
     /* Derived columns of the evaluator's rows (rc_eval.hip): pair k =
        (unary operator, source column) is column v + k, stored at
        d_derived + k * n; computed once and kept with the rows. */
+    RcOrderBufs order;
     std::vector<uint32_t> derived;
     float *d_derived = nullptr;
     int derived_cols = 0;   /* computed */
@@ -135,6 +166,11 @@ double rc_now_ms();
 int rc_dev_alloc(RcCtx *c, void **p, size_t bytes);
 void rc_dev_free(RcCtx *c, void *p);
 int rc_grow(RcCtx *c, void **p, size_t *cap, size_t need, size_t elem);
+int rc_rows_ensure_f64(RcCtx *c, RcRows *r);
+/* rc_order.hip: drop the cached target ranks (the f64 rows changed) / free
+   the stage buffers. */
+int rc_order_invalidate(RcCtx *c);
+void rc_order_free(RcCtx *c);
 int rc_grow_pinned(RcCtx *c, void **p, size_t *cap, size_t need);
 int rc_rows_upload(RcRows *r, const float *cols, const float *y, int n, int v);
 void rc_rows_free(RcRows *r);

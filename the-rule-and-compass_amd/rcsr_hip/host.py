@@ -98,6 +98,12 @@ class Population:
         return np.ascontiguousarray(prog), np.asarray(self.offsets, np.uint32)
 
 
+def _i128(hi, lo) -> int:
+    """Two's-complement 128-bit value of (hi, lo) u64 halves."""
+    u = (int(hi) << 64) | int(lo)
+    return u - (1 << 128) if u >> 127 else u
+
+
 class GpuFitReject(Enum):
     """fit.rs:958-983."""
     NO_CONSTANTS = "no_constants"
@@ -325,6 +331,66 @@ class GpuEvaluator:
         outs = [GpuFitOutput(outc[i, :plans[i].nconst].copy(), float(outsse[i]), int(outst[i]))
                 for i in range(n)]
         return outs, ms
+
+    # -- exact order, rank and maximum statistics ------------------------
+    def _stage_pack(self, plans, c, ntrials):
+        """Programs of `plans` and constants c (len(plans) * ntrials rows of
+        kmax; None: each plan's start constants for every trial)."""
+        words, off, pcount, c0p, kmax = pack_plans(plans, None)
+        if c is None:
+            c = np.repeat(c0p.reshape(len(plans), kmax), ntrials, axis=0)
+        c = np.ascontiguousarray(np.asarray(c, np.float64).reshape(len(plans) * ntrials, -1))
+        if c.shape[1] != kmax:
+            cc = np.zeros((c.shape[0], kmax))
+            w = min(kmax, c.shape[1])
+            cc[:, :w] = c[:, :w]
+            c = cc
+        return words, off, pcount, np.ascontiguousarray(c.reshape(-1)), kmax
+
+    def order_stat_batch(self, plans, what: int, c=None, ntrials: int = 1):
+        """rc_order_stat_score_batch: (values, status, ms), one per segment."""
+        words, off, pcount, c0p, kmax = self._stage_pack(plans, c, ntrials)
+        ns = len(plans) * ntrials
+        val = np.zeros(ns)
+        st = np.zeros(ns, np.int32)
+        ms = self._L.rc_order_stat_score_batch(
+            self.ctx, _L.ptr(words, _L.u32p), _L.ptr(off, _L.u32p), _L.ptr(pcount, _L.i32p),
+            len(plans), int(ntrials), kmax, _L.ptr(c0p, _L.f64p), int(what), _L.ptr(val, _L.f64p),
+            _L.ptr(st, _L.i32p))
+        return val, st, ms
+
+    def rank_score_batch(self, plans, c=None, ntrials: int = 1):
+        """rc_rank_score_batch: ([(sum AB, sum AA)], sum BB, status, ms)."""
+        words, off, pcount, c0p, kmax = self._stage_pack(plans, c, ntrials)
+        ns = len(plans) * ntrials
+        out = np.zeros(4 * ns, np.uint64)
+        bb = np.zeros(2, np.uint64)
+        st = np.zeros(ns, np.int32)
+        ms = self._L.rc_rank_score_batch(
+            self.ctx, _L.ptr(words, _L.u32p), _L.ptr(off, _L.u32p), _L.ptr(pcount, _L.i32p),
+            len(plans), int(ntrials), kmax, _L.ptr(c0p, _L.f64p), _L.ptr(out, _L.u64p),
+            _L.ptr(st, _L.i32p), _L.ptr(bb, _L.u64p))
+        sums = [(_i128(out[4 * s], out[4 * s + 1]), _i128(out[4 * s + 2], out[4 * s + 3]))
+                for s in range(ns)]
+        return sums, _i128(bb[0], bb[1]), st, ms
+
+    def rank_debug_b(self) -> Optional[np.ndarray]:
+        out = np.zeros(self.n, np.int64)
+        rc = self._L.rc_rank_debug_b(self.ctx, out.ctypes.data_as(_L.C.POINTER(_L.C.c_longlong)))
+        return None if rc != 0 else out
+
+    def max_scan_batch(self, plans, c=None, ntrials: int = 1, coords=None, levels=None):
+        """rc_max_scan_batch: ([RcMaxOut], ms)."""
+        words, off, pcount, c0p, kmax = self._stage_pack(plans, c, ntrials)
+        ns = len(plans) * ntrials
+        out = (_L.RcMaxOut * ns)()
+        co = None if coords is None else np.ascontiguousarray(coords, np.int32)
+        lv = None if levels is None else np.ascontiguousarray(levels, np.float64)
+        ms = self._L.rc_max_scan_batch(
+            self.ctx, _L.ptr(words, _L.u32p), _L.ptr(off, _L.u32p), _L.ptr(pcount, _L.i32p),
+            len(plans), int(ntrials), kmax, _L.ptr(c0p, _L.f64p), _L.ptr(co, _L.i32p),
+            _L.ptr(lv, _L.f64p), out)
+        return list(out), ms
 
     def fit_batch_hetero(self, plans, rowset, metric, iters: int, kmax=None):
         words, off, pcount, c0p, kmax = pack_plans(plans, kmax)

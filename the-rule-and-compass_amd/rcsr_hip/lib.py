@@ -38,6 +38,11 @@ class RcArenaReport(C.Structure):
                 ("fs_pinned_bytes", C.c_ulonglong)]
 
 
+class RcMaxOut(C.Structure):
+    _fields_ = [("vmax", C.c_double), ("lo", C.c_double), ("hi", C.c_double), ("row", C.c_int),
+                ("status", C.c_int)]
+
+
 class RcFitReport(C.Structure):
     _fields_ = [("asm_ms", C.c_double), ("trial_ms", C.c_double), ("exact_ms", C.c_double),
                 ("ctrl_ms", C.c_double), ("asm_launches", C.c_int),
@@ -92,6 +97,13 @@ SIGNATURES = {
     "rch_fit_plan": (C.c_int, [i32p, f64p, C.c_int, u32p, C.c_int, f64p, C.c_int, i32p]),
     "rch_fitvm_compile": (C.c_int, [u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, u32p,
                                     C.c_int, u32p, C.c_int, i32p]),
+    "rc_order_stat_score_batch": (C.c_float, [vp, u32p, u32p, i32p, C.c_int, C.c_int, C.c_int,
+                                              f64p, C.c_int, f64p, i32p]),
+    "rc_rank_score_batch": (C.c_float, [vp, u32p, u32p, i32p, C.c_int, C.c_int, C.c_int, f64p,
+                                        u64p, i32p, u64p]),
+    "rc_rank_debug_b": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
+    "rc_max_scan_batch": (C.c_float, [vp, u32p, u32p, i32p, C.c_int, C.c_int, C.c_int, f64p,
+                                      i32p, f64p, C.POINTER(RcMaxOut)]),
     "rch_eval_translate": (C.c_int, [u32p, u32p, C.c_int, C.c_int, u32p, C.c_int, u32p, C.c_int,
                                      u32p, C.c_int, i32p, i32p]),
 }
