@@ -92,6 +92,10 @@ def main():
     ap.add_argument("--eval-progs", type=int, default=4096)
     ap.add_argument("--no-c4", action="store_true",
                     help="skip the uneven-batch (autotuned vs fixed shapes) part")
+    ap.add_argument("--no-order", action="store_true",
+                    help="skip the exact order / rank / maximum stages part")
+    ap.add_argument("--order-only", action="store_true",
+                    help="only the exact order / rank / maximum stages (one JSON object)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device copy of 256 MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -109,6 +113,9 @@ def main():
 
     if args.eval_only:
         print(json.dumps(eval_part(args, host, workloads)), flush=True)
+        return
+    if args.order_only:
+        print(json.dumps(order_part(host, workloads)), flush=True)
         return
 
     t0 = time.time()
@@ -266,7 +273,51 @@ def main():
     # squared / L1): the built-in launch shapes against the autotuned ones.
     if not args.no_c4:
         result["c4_shapes"] = c4_part(host, workloads)
+    # Exact order / rank / maximum stages on the C3 rows.
+    if not args.no_order:
+        result["order_stages"] = order_part(host, workloads)
     print(json.dumps(result), flush=True)
+
+
+def order_part(host, workloads, ncand=64, ntrials=4):
+    """The exact f64 stages over the C3 rows: 64 candidates x 4 constant
+    vectors (256 segments of N = 262144 rows)."""
+    import numpy as np
+    ts = time.perf_counter()
+    wl = workloads.config3(n=262144, ncand=ncand, iters=30)
+    plans = [host.gpu_fit_plan(t) for t in wl.trees]
+    kmax = max(p.nconst for p in plans)
+    c = np.zeros((ncand * ntrials, kmax))
+    for i, p in enumerate(plans):
+        for t in range(ntrials):
+            c[i * ntrials + t, :p.nconst] = np.asarray(p.c0) * (1.0 + 0.01 * t)
+    ev = host.GpuEvaluator(wl.cols32(), wl.n, wl.v, wl.y.astype(np.float32))
+    assert ev.set_fit_data(wl.cols64(), wl.y)
+    log(f"order workload ready in {time.perf_counter() - ts:.1f}s: {ncand} x {ntrials} segments "
+        f"of N={wl.n}")
+    segrows = float(ncand * ntrials * wl.n)
+    out = {"workload": f"{ncand} C3 candidates x {ntrials} constant vectors, N={wl.n}, f64",
+           "segment_rows": segrows}
+    first = ev.rank_score_batch(plans, c, ntrials)[3]
+    runs = {
+        "rank_sums": lambda: ev.rank_score_batch(plans, c, ntrials)[3],
+        "median": lambda: ev.order_stat_batch(plans, 0, c, ntrials)[2],
+        "interquartile_mean": lambda: ev.order_stat_batch(plans, 1, c, ntrials)[2],
+        "max_scan": lambda: ev.max_scan_batch(plans, c, ntrials)[1],
+        "max_scan_with_level": lambda: ev.max_scan_batch(
+            plans, c, ntrials, coords=[0] * (ncand * ntrials), levels=[1.0] * (ncand * ntrials))[1],
+    }
+    for name, f in runs.items():
+        f()
+        ms = min(f() for _ in range(3))
+        if ms < 0:
+            raise RuntimeError(f"{name} failed")
+        out[name] = {"ms": ms, "segment_rows_per_s": segrows / (ms * 1e-3)}
+    out["rank_sums_first_call_ms"] = first
+    ev.close()
+    log("order stages: " + ", ".join(f"{k} {v['ms']:.1f} ms" for k, v in out.items()
+                                     if isinstance(v, dict)))
+    return out
 
 
 def c4_part(host, workloads):
