@@ -60,6 +60,11 @@ def lib():
         L.or_implicit_fit_batch.restype = None
         L.or_implicit_fit_batch.argtypes = [_ip, _dp, _ip, C.c_int, _dp, C.c_int, C.c_int, _dp,
                                             C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.or_corr_fit.restype = C.c_double
+        L.or_corr_fit.argtypes = [_ip, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp, C.c_int,
+                                  C.c_int, C.c_int, C.c_int, _dp, _ip]
+        L.or_corr_score.restype = C.c_double
+        L.or_corr_score.argtypes = [_ip, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp, C.c_int]
         L.or_predict_flat.restype = None
         L.or_predict_flat.argtypes = [_ip, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp]
         L.or_median_abs.restype = C.c_double
@@ -301,3 +306,25 @@ def predict_flat(tree, X) -> np.ndarray:
     out = np.zeros(X.shape[0])
     lib().or_predict_flat(_i(ops), _d(vals), len(ops), _d(X), X.shape[0], X.shape[1], _d(out))
     return out
+
+
+def corr_fit(tree, X, y, w=None, iters=8, metric=2, gauge=(-1, -1)):
+    """The device correlation arm restated: (objective, constants, status)."""
+    ops, vals = _tree(tree)
+    X = _rows(X)
+    y = np.ascontiguousarray(y, np.float64)
+    ww = _w(w)
+    out = np.zeros(max(nconst(tree), 1))
+    st = C.c_int()
+    obj = lib().or_corr_fit(_i(ops), _d(vals), len(ops), _d(X), X.shape[0], X.shape[1], _d(y),
+                            _d(ww), int(iters), int(metric), int(gauge[0]), int(gauge[1]),
+                            _d(out), C.byref(st))
+    return obj, out[:nconst(tree)], st.value
+
+
+def corr_score(tree, X, y, w=None, metric=2) -> float:
+    ops, vals = _tree(tree)
+    X = _rows(X)
+    y = np.ascontiguousarray(y, np.float64)
+    return lib().or_corr_score(_i(ops), _d(vals), len(ops), _d(X), X.shape[0], X.shape[1], _d(y),
+                               _d(_w(w)), int(metric))

@@ -1215,6 +1215,316 @@ void or_implicit_fit_batch(const int *ops, const double *vals, const int *tree_o
     });
 }
 
+} // extern "C"
+
+namespace {
+
+/* ---- the correlation fit arm (metrics 2 and 3) ------------------------- */
+
+/* The equilibrated, damped Cholesky step of the device fitter (eval.cu:
+   1874-1949): jtj packed lower triangle (p x p), delta = -(D^-1/2 (A + lambda
+   diag A) D^-1/2 + 1e-12)^-1 jtr in scaled form.  Returns 1 unusable, 4
+   solved but rank-deficient (ln det below the floor), 0 solved. */
+int eq_solve(const std::vector<double> &jtj, const std::vector<double> &jtr, double lambda, int p,
+             std::vector<double> &delta)
+{
+    const double LN_DET_FLOOR = -27.631021115928547;
+    std::vector<double> dsc(p), L((size_t)p * p, 0.0);
+    for (int i = 0; i < p; ++i) {
+        const double aii = jtj[i * (i + 1) / 2 + i];
+        if (!(aii > 0.0) || !std::isfinite(aii)) return 1;
+        dsc[i] = std::sqrt(aii);
+    }
+    bool ok = true;
+    double sum_ln = 0.0;
+    for (int i = 0; i < p && ok; ++i) {
+        for (int j = 0; j <= i; ++j) {
+            double s = jtj[i * (i + 1) / 2 + j] / (dsc[i] * dsc[j]);
+            if (!std::isfinite(s)) return 1;
+            for (int q = 0; q < j; ++q) s = std::fma(-L[i * p + q], L[j * p + q], s);
+            if (i == j) {
+                if (!(s > 0.0) || !std::isfinite(s)) { ok = false; break; }
+                L[i * p + i] = std::sqrt(s);
+                sum_ln = std::fma(0.5, std::log(s), sum_ln);
+            } else {
+                L[i * p + j] = s / L[j * p + j];
+            }
+        }
+    }
+    bool rank = !ok;
+    if (ok) {
+        const double ln_det = 2.0 * sum_ln;
+        if (!std::isfinite(ln_det) || ln_det < LN_DET_FLOOR) rank = true;
+    }
+    for (int i = 0; i < p; ++i) {
+        for (int j = 0; j <= i; ++j) {
+            double s = jtj[i * (i + 1) / 2 + j] / (dsc[i] * dsc[j]);
+            if (i == j) s = std::fma(s, 1.0 + lambda, 1e-12);
+            for (int q = 0; q < j; ++q) s = std::fma(-L[i * p + q], L[j * p + q], s);
+            if (i == j) {
+                if (!(s > 1e-18) || !std::isfinite(s)) return 1;
+                L[i * p + i] = std::sqrt(s);
+            } else {
+                L[i * p + j] = s / L[j * p + j];
+            }
+        }
+    }
+    std::vector<double> rhs(p);
+    for (int i = 0; i < p; ++i) rhs[i] = -jtr[i] / dsc[i];
+    for (int i = 0; i < p; ++i) {
+        double s = rhs[i];
+        for (int q = 0; q < i; ++q) s = std::fma(-L[i * p + q], rhs[q], s);
+        rhs[i] = s / L[i * p + i];
+    }
+    delta.assign(p, 0.0);
+    for (int i = p - 1; i >= 0; --i) {
+        double s = rhs[i];
+        for (int q = i + 1; q < p; ++q) s = std::fma(-L[q * p + i], delta[q], s);
+        delta[i] = s / L[i * p + i];
+    }
+    for (int i = 0; i < p; ++i) delta[i] /= dsc[i];
+    return rank ? 4 : 0;
+}
+
+struct CorrState {
+    std::vector<double> jtj, jtr;
+    double sff = 0.0, syy = 0.0, rho = 0.0, score = 1.0;
+    bool bad = false;
+};
+
+/* correlation.rs moments / ft_corr_assemble (eval.cu:2009-2177): weighted
+   moments over the rows with positive weight (a non-finite prediction,
+   target or derivative marks the pass bad), the rho gradient and its
+   Gauss-Newton curvature. */
+void corr_assemble(const Flat &f, const Data &D, const std::vector<double> &c, int metric,
+                   CorrState &S)
+{
+    const int p = f.nconst;
+    const int m = (int)f.op.size();
+    std::vector<double> v(m), adj(m), g(p);
+    std::vector<double> C((size_t)p * (p + 1) / 2, 0.0), sg(p, 0.0), sgf(p, 0.0), sgy(p, 0.0);
+    double W = 0.0, sf = 0.0, sf2 = 0.0, sft = 0.0, st = 0.0, st2 = 0.0;
+    S.bad = false;
+    for (int i = 0; i < D.n; ++i) {
+        const double w = D.w ? D.w[i] : 1.0;
+        if (!(w > 0.0)) continue;
+        eval_row(f, D.row(i), D.nv, c.data(), v);
+        const double fv = v[f.root];
+        const double t = D.y[i];
+        bool ok = std::isfinite(fv) && std::fabs(fv) <= 1e30 && std::isfinite(t);
+        backward_impl(f, v, adj, g.data(), p, nullptr, 0);
+        for (int j = 0; j < p; ++j)
+            if (!std::isfinite(g[j])) ok = false;
+        if (!ok) {
+            S.bad = true;
+            continue;
+        }
+        int k = 0;
+        for (int a = 0; a < p; ++a)
+            for (int b = 0; b <= a; ++b, ++k) C[k] += (w * g[a]) * g[b];
+        for (int j = 0; j < p; ++j) {
+            sg[j] += w * g[j];
+            sgf[j] += (w * fv) * g[j];
+            sgy[j] += (w * t) * g[j];
+        }
+        W += w;
+        sf += w * fv;
+        sf2 += (w * fv) * fv;
+        st += w * t;
+        sft += (w * t) * fv;
+        st2 += (w * t) * t;
+    }
+    S.jtj = C;
+    S.jtr.assign(p, 0.0);
+    double sff = 0.0, syy = 0.0, rho = 0.0;
+    if (W > 0.0) {
+        const double mf = sf / W, mt = st / W;
+        sff = sf2 - W * mf * mf;
+        syy = st2 - W * mt * mt;
+        const double sfy = sft - W * mf * mt;
+        if (sff > 0.0 && syy > 0.0) rho = sfy / std::sqrt(sff * syy);
+        std::vector<double> A(p), B(p), gbar(p);
+        for (int j = 0; j < p; ++j) {
+            gbar[j] = sg[j] / W;
+            A[j] = sgf[j] - W * gbar[j] * mf;
+            B[j] = sgy[j] - W * gbar[j] * mt;
+        }
+        if (sff > 0.0 && syy > 0.0) {
+            const double root = std::sqrt(sff * syy);
+            for (int j = 0; j < p; ++j) S.jtr[j] = B[j] / root - rho * A[j] / sff;
+            int k = 0;
+            for (int a = 0; a < p; ++a)
+                for (int b = 0; b <= a; ++b, ++k)
+                    S.jtj[k] = ((C[k] - W * gbar[a] * gbar[b]) - A[a] * A[b] / sff) / sff;
+        } else {
+            int k = 0;
+            for (int a = 0; a < p; ++a)
+                for (int b = 0; b <= a; ++b, ++k) S.jtj[k] = (a == b) ? 1.0 : 0.0;
+        }
+    }
+    S.sff = sff;
+    S.syy = syy;
+    S.rho = rho;
+    S.score = metric == 3 ? 1.0 - rho : 1.0 - std::fabs(rho);
+    if (!std::isfinite(S.score) || !std::isfinite(sff) || !std::isfinite(syy)) S.bad = true;
+}
+
+/* The objective at c (correlation.rs objective of the weighted moments); a
+   non-finite row scores 1e30. */
+double corr_score(const Flat &f, const Data &D, const std::vector<double> &c, int metric)
+{
+    std::vector<double> v(f.op.size());
+    double W = 0.0, sf = 0.0, sf2 = 0.0, sft = 0.0, st = 0.0, st2 = 0.0;
+    for (int i = 0; i < D.n; ++i) {
+        const double w = D.w ? D.w[i] : 1.0;
+        if (!(w > 0.0)) continue;
+        eval_row(f, D.row(i), D.nv, c.data(), v);
+        const double fv = v[f.root];
+        const double t = D.y[i];
+        if (!std::isfinite(fv) || std::fabs(fv) > 1e30 || !std::isfinite(t)) return 1e30;
+        W += w;
+        sf += w * fv;
+        sf2 += w * fv * fv;
+        sft += w * fv * t;
+        st += w * t;
+        st2 += w * t * t;
+    }
+    if (!(W > 0.0)) return 1e30;
+    const double mf = sf / W, mt = st / W;
+    const double sff = sf2 - W * mf * mf, syy = st2 - W * mt * mt, sfy = sft - W * mf * mt;
+    if (!(sff > 0.0) || !(syy > 0.0)) return 1.0;
+    const double rho = sfy / std::sqrt(sff * syy);
+    return metric == 3 ? 1.0 - rho : 1.0 - std::fabs(rho);
+}
+
+/* One step: the assembled system, objective gradient -sigma * d rho, gauge
+   rows pinned (eval.cu:2244-2262), the equilibrated solve, the clamp. */
+bool corr_step(const CorrState &S, const std::vector<double> &c, int p, double lambda, int metric,
+               int gs, int go, std::vector<double> &trial, bool &rank)
+{
+    std::vector<double> jtj = S.jtj, jtr = S.jtr;
+    const double sigma = metric == 3 ? 1.0 : (S.rho >= 0.0 ? 1.0 : -1.0);
+    for (int j = 0; j < p; ++j) jtr[j] = -sigma * jtr[j];
+    for (int i = 0; i < p; ++i) {
+        const bool gi = gs == i || go == i;
+        for (int j = 0; j <= i; ++j) {
+            const bool gj = gs == j || go == j;
+            if (gi || gj) jtj[i * (i + 1) / 2 + j] = (i == j) ? 1.0 : 0.0;
+        }
+        if (gi) jtr[i] = 0.0;
+    }
+    std::vector<double> delta;
+    const int rc = eq_solve(jtj, jtr, lambda, p, delta);
+    if (rc == 1) return false;
+    if (rc == 4) rank = true;
+    trial.resize(p);
+    for (int i = 0; i < p; ++i) trial[i] = std::fmin(1e6, std::fmax(-1e6, c[i] + delta[i]));
+    return true;
+}
+
+/* The device correlation arm (eval.cu:2320-2448, each step on a freshly
+   masked system): gauge renormalisation, LM with the damping schedule of
+   the other arms, four undamped polish steps, status labels. */
+double corr_fit(const Flat &f, const Data &D, std::vector<double> &c, int iters, int metric,
+                int gs, int go, int &status)
+{
+    const int p = f.nconst;
+    if (gs >= 0 && gs < p && std::isfinite(c[gs]) && c[gs] != 0.0) c[gs] = c[gs] > 0.0 ? 1.0 : -1.0;
+    if (go >= 0 && go < p) c[go] = 0.0;
+    status = 0;
+    double best = 1.0;
+    bool start = false, improved = false, rank = false;
+    CorrState S;
+    if (iters > 0) {
+        corr_assemble(f, D, c, metric, S);
+        start = !S.bad;
+        if (S.bad) {
+            status = 2;
+        } else if (!(S.sff > 0.0) || !(S.syy > 0.0)) {
+            best = 1.0;
+        } else if (S.sff < 1e-24 * S.syy) {
+            status = 1;
+        } else {
+            best = S.score;
+            double lambda = 1e-3;
+            bool assembled = true;
+            for (int it = 0; it < iters && status == 0; ++it) {
+                if (!assembled) {
+                    corr_assemble(f, D, c, metric, S);
+                    assembled = true;
+                    if (S.bad) { status = 2; break; }
+                    if (S.sff < 1e-24 * S.syy) { status = 1; break; }
+                }
+                std::vector<double> trial;
+                if (!corr_step(S, c, p, lambda, metric, gs, go, trial, rank)) { status = 1; break; }
+                const double t = corr_score(f, D, trial, metric);
+                if (t < best) {
+                    c = trial;
+                    best = t;
+                    lambda = std::fmax(lambda / 8.0, 1e-12);
+                    assembled = false;
+                    improved = true;
+                    if (best <= 0.0) break;
+                } else {
+                    lambda *= 6.0;
+                    if (lambda > 1e9) break;
+                }
+            }
+            for (int it = 0; it < 4 && status == 0 && best > 0.0; ++it) {
+                corr_assemble(f, D, c, metric, S);
+                if (S.bad) { status = 2; break; }
+                std::vector<double> trial;
+                if (!corr_step(S, c, p, 0.0, metric, gs, go, trial, rank)) break;
+                const double t = corr_score(f, D, trial, metric);
+                if (t < best) {
+                    c = trial;
+                    best = t;
+                    improved = true;
+                } else {
+                    break;
+                }
+            }
+        }
+    } else {
+        best = corr_score(f, D, c, metric);
+    }
+    if (status == 1 || status == 2) status = !start ? 7 : (improved ? 5 : 6);
+    else if (status == 0 && rank) status = 4;
+    return best;
+}
+
+} // namespace (correlation)
+
+extern "C" {
+
+double or_corr_fit(const int *ops, const double *vals, int nn, const double *x, int n, int nv,
+                   const double *y, const double *w, int iters, int metric, int gauge_scale,
+                   int gauge_offset, double *out_c, int *status)
+{
+    Node t;
+    *status = -1;
+    if (!build_tree(ops, vals, nn, t)) return NaN;
+    Flat f = flatten(t);
+    std::vector<double> c;
+    collect_consts(t, c);
+    Data D{x, n, nv, y, w};
+    const double best = corr_fit(f, D, c, iters, metric, gauge_scale, gauge_offset, *status);
+    for (size_t i = 0; i < c.size(); ++i) out_c[i] = c[i];
+    return best;
+}
+
+double or_corr_score(const int *ops, const double *vals, int nn, const double *x, int n, int nv,
+                     const double *y, const double *w, int metric)
+{
+    Node t;
+    if (!build_tree(ops, vals, nn, t)) return NaN;
+    Flat f = flatten(t);
+    std::vector<double> c;
+    collect_consts(t, c);
+    Data D{x, n, nv, y, w};
+    return corr_score(f, D, c, metric);
+}
+
 /* ---- order, rank and maximum statistics of residual vectors ----------- */
 
 /* metrics::median_abs (metrics.rs:1193-1205): the middle |r| of the sorted

@@ -113,6 +113,18 @@ void or_implicit_fit_batch(const int *ops, const double *vals,
                            int n, int nv, const double *w, int iters, int kmax,
                            int threads, double *out_c, double *out_loss);
 
+/* The device correlation fit arm (metric 2: 1 - |rho|, 3: 1 - rho) with
+   the affine gauge pair (slot indices, -1 for none), restated in f64 from
+   eval.cu:2320-2448 with each step's system masked afresh: returns the
+   objective, writes the constants (slot order) and the status label. */
+double or_corr_fit(const int *ops, const double *vals, int nn, const double *x, int n, int nv,
+                   const double *y, const double *w, int iters, int metric, int gauge_scale,
+                   int gauge_offset, double *out_c, int *status);
+
+/* The weighted correlation objective of the tree's own constants. */
+double or_corr_score(const int *ops, const double *vals, int nn, const double *x, int n, int nv,
+                     const double *y, const double *w, int metric);
+
 /* Order, rank and maximum statistics of residual / prediction vectors. */
 
 /* metrics::median_abs (metrics.rs:1193-1205). */

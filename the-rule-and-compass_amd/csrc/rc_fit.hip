@@ -1592,11 +1592,17 @@ float rc_fit_batch(void *ctx, const unsigned int *prog, const unsigned int *off,
                    int nprog, int kmax, const double *c0, const int *gauge, int iters, int metric,
                    double *outc, double *outsse, int *outstatus)
 {
-    (void)gauge;
     RcCtx *c = (RcCtx *)ctx;
     if (!c || !prog || !off || !pcount || !c0 || nprog <= 0 || kmax <= 0 || iters < 0) return -1.0f;
     if (!outc || !outsse || !outstatus) return -1.0f;
     if (kmax > FT_KMAX) return -1.0f;
+    if (metric == RC_LOSS_CORRELATION || metric == RC_LOSS_CORRELATION_SIGNED) {
+        /* The correlation arm with the gauge pairs (eval.cu:2320-2448). */
+        if (rc_corr_run(c, prog, off, pcount, nprog, kmax, c0, gauge, iters, metric, outc, outsse,
+                        outstatus) != 0)
+            return -1.0f;
+        return (float)c->stats.kernel_ms;
+    }
     return fit_run(c, prog, off, pcount, nprog, kmax, c0, nullptr, nullptr, metric, iters, outc,
                    outsse, outstatus);
 }

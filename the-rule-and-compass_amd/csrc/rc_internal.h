@@ -170,6 +170,10 @@ int rc_rows_ensure_f64(RcCtx *c, RcRows *r);
 /* rc_order.hip: drop the cached target ranks (the f64 rows changed) / free
    the stage buffers. */
 int rc_order_invalidate(RcCtx *c);
+/* rc_corr.hip: the correlation arm of rc_fit_batch (metrics 2 and 3). */
+int rc_corr_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const int *pcount, int nprog,
+                int kmax, const double *c0, const int *gauge, int iters, int metric, double *outc,
+                double *outsse, int *outstatus);
 void rc_order_free(RcCtx *c);
 int rc_grow_pinned(RcCtx *c, void **p, size_t *cap, size_t need);
 int rc_rows_upload(RcRows *r, const float *cols, const float *y, int n, int v);

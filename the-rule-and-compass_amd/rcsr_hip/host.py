@@ -313,8 +313,10 @@ class GpuEvaluator:
 
     # -- fit --------------------------------------------------------------
     def fit_batch(self, plans: Sequence[GpuFitPlan], iters: int, metric: int = 0,
-                  kmax: Optional[int] = None, c0=None):
-        """gpu.rs:1909: returns ([GpuFitOutput], ms); ms < 0 discards all."""
+                  kmax: Optional[int] = None, c0=None, gauge=None):
+        """gpu.rs:1909: returns ([GpuFitOutput], ms); ms < 0 discards all.
+        gauge: per plan the (scale, offset) constant slots the correlation
+        metrics (2, 3) freeze, -1 for none."""
         words, off, pcount, c0p, kmax = pack_plans(plans, kmax)
         if c0 is not None:
             c0p = np.ascontiguousarray(c0, np.float64)
@@ -322,7 +324,8 @@ class GpuEvaluator:
         outc = np.zeros((n, kmax))
         outsse = np.zeros(n)
         outst = np.zeros(n, np.int32)
-        gauge = np.full(2 * n, -1, np.int32)
+        gauge = (np.full(2 * n, -1, np.int32) if gauge is None
+                 else np.ascontiguousarray(np.asarray(gauge, np.int32).reshape(-1)))
         ms = self._L.rc_fit_batch(self.ctx, _L.ptr(words, _L.u32p), _L.ptr(off, _L.u32p),
                                   _L.ptr(pcount, _L.i32p), n, kmax, _L.ptr(c0p, _L.f64p),
                                   _L.ptr(gauge, _L.i32p), int(iters), int(metric),
