@@ -136,10 +136,13 @@ int rc_pool_hash(void *ctx, int which, int nprog, int width, unsigned long long 
 /* eval.cu:4001 rc_fit_batch: batched LM / IRLS constant fitting.  prog holds
    post-order 4-word records [op, kid0, kid1, slot] (fit.rs:1101-1196),
    off nprog+1 word offsets, pcount the constant count of each candidate,
-   c0 nprog x kmax start constants; gauge is accepted for ABI compatibility
-   (the correlation arm is not built).  metric: 0 squared, 1 log-squashed,
-   4 absolute.  Outputs: constants (nprog x kmax), the fitted objective
-   sum w*rho(r) in f64, and the status.  kmax > 20 -> -1. */
+   c0 nprog x kmax start constants; gauge (may be NULL) holds per candidate
+   the (scale, offset) constant slots of its outer affine pair, -1 for none.
+   metric: 0 squared, 1 log-squashed, 4 absolute (objective sum w*rho(r));
+   2 correlation 1 - |rho|, 3 signed correlation 1 - rho (eval.cu:2320-2448:
+   the gauge pair is renormalised to (sign(a), 0) and frozen; objective the
+   weighted Pearson form).  Outputs: constants (nprog x kmax), the fitted
+   objective in f64, and the status.  kmax > 20 -> -1. */
 float rc_fit_batch(void *ctx, const unsigned int *prog, const unsigned int *off,
                    const int *pcount, int nprog, int kmax, const double *c0, const int *gauge,
                    int iters, int metric, double *outc, double *outsse, int *outstatus);
