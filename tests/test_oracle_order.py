@@ -102,3 +102,12 @@ def test_max_scan_maximum_row_and_level_interval():
     assert st == 0 and lo == math.inf and hi == -math.inf
     vm, row, _, _, st = oracle.max_scan([1.0, np.nan])
     assert st == 2 and row == -1 and math.isnan(vm)
+
+
+def test_flat_predictions_equal_the_tree_evaluator_on_fit_programs():
+    from rcsr_hip import workloads
+    wl = workloads.config3(n=300, ncand=60, iters=8)
+    for t in wl.trees:
+        a = oracle.predict_flat(t, wl.x)
+        b = oracle.predict(t, wl.x)
+        assert np.array_equal(a, b, equal_nan=True), str(t)
