@@ -398,3 +398,51 @@ def test_classification_scores_match_an_f64_reducer(data, kind):
     vals, st, _ = ev.score_words(prog[:int(off[1])], off[:2], kind)
     assert st[0] == 2
     ev.close()
+
+
+def _wide_tree(rng, nvars, size):
+    """A random tree over a wider operator set (unary chains of inputs recur,
+    so derived columns of derived columns are admitted)."""
+    un = [SIN, COS, EXP, LOG, SQRT, TANH, NEG, SQUARE, ABS]
+    bi = [ADD, SUB, MUL, DIV, MIN, MAX]
+    if size <= 1:
+        return const(float(rng.normal())) if rng.random() < 0.3 else var(int(rng.integers(nvars)))
+    if size == 2 or rng.random() < 0.35:
+        return node(un[int(rng.integers(len(un)))], _wide_tree(rng, nvars, size - 1))
+    left = int(rng.integers(1, size - 1))
+    return node(bi[int(rng.integers(len(bi)))], _wide_tree(rng, nvars, left),
+                _wide_tree(rng, nvars, size - 1 - left))
+
+
+def test_many_inputs_and_shared_subtrees_match_the_oracle():
+    """12 input columns (past the 8 a fit program may use), 600 programs whose
+    unary-of-input subtrees recur (derived columns, some of them derived from
+    derived ones) and input-free subtrees (hoisted): predictions against the
+    f64 oracle, and every program alone in a fresh context (no derived column
+    admitted) gives the same bits."""
+    rng = np.random.default_rng(31)
+    n, v = 2500, 12
+    x = rng.normal(0.0, 1.0, size=(n, v)).astype(np.float32).astype(np.float64)
+    y = np.sin(x[:, 0]).astype(np.float32).astype(np.float64)
+    cols = x.T.astype(np.float32).reshape(-1)
+    ev = host.GpuEvaluator(cols, n, v, y.astype(np.float32))
+    trees = [_wide_tree(rng, v, int(rng.integers(2, 30))) for _ in range(600)]
+    pred = ev.predict(trees)
+    assert pred is not None
+    ok_rows = all_rows = 0
+    for i in range(0, 600, 10):
+        want = oracle.predict(trees[i], x)
+        fin = np.isfinite(want) & (np.abs(want) < 1e6) & np.isfinite(pred[i])
+        if fin.sum():
+            e = np.abs(pred[i][fin] - want[fin]) / (1.0 + np.abs(want[fin]))
+            ok_rows += int(np.sum(e < 1e-4))
+            all_rows += int(fin.sum())
+    assert all_rows > 0 and ok_rows / all_rows > 0.995
+    for i in range(0, 600, 60):
+        one = host.GpuEvaluator(cols, n, v, y.astype(np.float32))
+        q = one.predict([trees[i]])[0]
+        one.close()
+        assert np.array_equal(np.isnan(q), np.isnan(pred[i])), str(trees[i])
+        fin = ~np.isnan(q)
+        assert np.array_equal(q[fin].view(np.uint32), pred[i][fin].view(np.uint32)), str(trees[i])
+    ev.close()
