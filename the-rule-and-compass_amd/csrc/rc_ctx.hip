@@ -16,11 +16,32 @@
 #include <string.h>
 #include <time.h>
 
+#include <algorithm>
+#include <thread>
+
 double rc_now_ms()
 {
     struct timespec ts;
     clock_gettime(CLOCK_MONOTONIC, &ts);
     return (double)ts.tv_sec * 1e3 + (double)ts.tv_nsec / 1e6;
+}
+
+void rc_parallel_for(int n, int min_chunk, const std::function<void(int, int)> &body)
+{
+    if (n <= 0) return;
+    const unsigned hw = std::thread::hardware_concurrency();
+    int nt = (int)std::min<unsigned>(hw ? hw : 1u, 8u);
+    nt = std::max(1, std::min(nt, n / std::max(1, min_chunk)));
+    if (nt == 1) {
+        body(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    pool.reserve((size_t)nt - 1);
+    for (int t = 1; t < nt; ++t)
+        pool.emplace_back(body, (int)((long long)n * t / nt), (int)((long long)n * (t + 1) / nt));
+    body(0, (int)((long long)n / nt));
+    for (std::thread &th : pool) th.join();
 }
 
 int rc_dev_alloc(RcCtx *c, void **p, size_t bytes)

@@ -1006,6 +1006,7 @@ static FtProg translate_fit(const uint32_t *w, int words, int p, int nvars, bool
         return out;
     }
     std::vector<int> st;
+    st.reserve((size_t)m);
     std::vector<char> onpath(m, 0);
     bool unsupported = false;
     for (int i = 0; i < m; ++i) {
@@ -1044,6 +1045,8 @@ static FtProg translate_fit(const uint32_t *w, int words, int p, int nvars, bool
     }
     /* Forward stream, stack depth and tape size. */
     int depth = 0, tape = 0;
+    out.fwd.reserve((size_t)m);
+    out.rev.reserve((size_t)m);
     for (int i = 0; i < m; ++i) {
         const uint32_t op = w[4 * i], k0 = w[4 * i + 1], k1 = w[4 * i + 2], slot = w[4 * i + 3];
         uint32_t fl = 0, arg = 0;
@@ -1182,6 +1185,41 @@ static bool prog_is_big(const FtProg &p)
            p.tape_r + 1 > SMALL_TAPE;
 }
 
+/* One candidate's fit-VM streams (word pairs, trimmed to length). */
+struct FtVmCode {
+    int vorc = -1, vmrc = -1;
+    bool vm4 = false;
+    std::vector<uint32_t> vf, vr, vt;
+};
+
+/* Compile a candidate for the fit VM: the value-pass stream (value-only
+   form, inputs in U, 8-slot file) and the forward/reverse pair, at 4 rows
+   per lane when the program fits that block's files (an 8-slot B file, one
+   saved adjoint, and a value-pass stream for the f64 polish), else at 2 rows
+   per lane, whose block has the 2-row column stride.  A stream that does not
+   compile leaves its code nonzero: the candidate then runs on the general
+   interpreter. */
+static void vm_compile(FtVmCode &o, const uint32_t *rec, int words, int p, int nv, int tune_rows)
+{
+    uint32_t vf[2 * 4 * FT_MAX_NODES], vr[2 * 8 * FT_MAX_NODES], vt[2 * 4 * FT_MAX_NODES];
+    int info[6] = {0, 0, 0, 0, 0, 0}, tinfo[6] = {0, 0, 0, 0, 0, 0};
+    const int nf = (int)(sizeof(vf) / 4), nr = (int)(sizeof(vr) / 4), ntv = (int)(sizeof(vt) / 4);
+    o.vorc = rch_fitvm_compile(rec, words, p, nv, VM_XS(4) * 4,
+                               VMC_INPUTS_IN_U | VMC_VALUE_ONLY | VMC_SMALL_U, vt, ntv, nullptr, 0,
+                               tinfo);
+    o.vmrc = rch_fitvm_compile(rec, words, p, nv, VM_XS(4) * 4, VMC_INPUTS_IN_U, vf, nf, vr, nr,
+                               info);
+    o.vm4 = o.vmrc == 0 && info[3] <= 8 && info[4] <= 1 && o.vorc == 0 && tune_rows != 2;
+    if (o.vmrc == 0 && !o.vm4)
+        o.vmrc = rch_fitvm_compile(rec, words, p, nv, VM_XS(2) * 4, VMC_INPUTS_IN_U, vf, nf, vr,
+                                   nr, info);
+    if (o.vorc == 0) o.vt.assign(vt, vt + 2 * tinfo[0]);
+    if (o.vmrc == 0) {
+        o.vf.assign(vf, vf + 2 * info[0]);
+        o.vr.assign(vr, vr + 2 * info[1]);
+    }
+}
+
 /* The shared driver of rc_fit_batch / rc_fit_batch_hetero. */
 static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const int *pcount,
                      int nprog, int kmax, const double *c0, const int *rowset_in,
@@ -1203,13 +1241,25 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         rowset[i] = rs;
         metric[i] = mt;
         if (mt == RC_LOSS_ABSOLUTE) any_irls = true;
-        const int words = (int)(off[i + 1] - off[i]);
-        if (pcount[i] < 1 || pcount[i] > kmax) {
-            progs[i].pre_status = 1;
-            continue;
-        }
-        progs[i] = translate_fit(prog + off[i], words, pcount[i], c->sets[rs].v, true, false);
     }
+    /* Per-candidate translation and fit-VM compilation are independent:
+       they run on host threads, and the streams are laid out in candidate
+       order afterwards. */
+    const bool vm_pass = f->mode != 1;
+    std::vector<FtVmCode> vmc(vm_pass ? (size_t)nprog : 0);
+    rc_parallel_for(nprog, 32, [&](int lo, int hi) {
+        for (int i = lo; i < hi; ++i) {
+            const int words = (int)(off[i + 1] - off[i]);
+            if (pcount[i] < 1 || pcount[i] > kmax) {
+                progs[i].pre_status = 1;
+                continue;
+            }
+            const int nv = c->sets[rowset[i]].v;
+            progs[i] = translate_fit(prog + off[i], words, pcount[i], nv, true, false);
+            if (vm_pass && progs[i].pre_status < 0)
+                vm_compile(vmc[i], prog + off[i], words, pcount[i], nv, c->tune_fit_rows);
+        }
+    });
     for (int s = 0; s < c->nsets; ++s)
         if (rc_rows_ensure_f64(c, &c->sets[s]) != 0) return -1.0f;
 
@@ -1236,8 +1286,6 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     std::vector<int> list[NA];
     std::vector<int> tlist[NT];
     int maxK = 2;
-    std::vector<uint32_t> vf(2 * 4 * FT_MAX_NODES), vr(2 * 8 * FT_MAX_NODES),
-        vt(2 * 4 * FT_MAX_NODES);
     const int tile_rows = c->tune_fit_tile > 0 ? c->tune_fit_tile : TILE_ROWS;
     for (int i = 0; i < nprog; ++i) {
         FtCand &d = cands[i];
@@ -1248,37 +1296,17 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         /* A host-decided candidate keeps k = 0: its outputs are c0 verbatim. */
         d.k = (pre[i] >= 0) ? 0 : pcount[i];
         if (pre[i] >= 0) continue;
-        const int words = (int)(off[i + 1] - off[i]);
-        int info[6] = {0, 0, 0, 0, 0, 0};
-        const int nv = c->sets[d.rowset].v;
-        /* The value-pass stream: the fit VM's value-only form (inputs in U,
-           8-slot file) when it fits, else the general interpreter's (it has
-           no reverse stream, so no column stride). */
-        int tinfo[6] = {0, 0, 0, 0, 0, 0};
-        const int vorc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, VM_XS(4) * 4,
-                                           VMC_INPUTS_IN_U | VMC_VALUE_ONLY | VMC_SMALL_U,
-                                           vt.data(), (int)vt.size(), nullptr, 0, tinfo);
-        int vmrc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, VM_XS(4) * 4,
-                                     VMC_INPUTS_IN_U, vf.data(), (int)vf.size(), vr.data(),
-                                     (int)vr.size(), info);
-        /* 4 rows per lane: an 8-slot B file, one saved adjoint and, for the
-           f64 polish, the value-pass stream.  Otherwise 2 rows per lane,
-           whose block has the 2-row column stride. */
-        const bool vm4 = vmrc == 0 && info[3] <= 8 && info[4] <= 1 && vorc == 0 &&
-                         c->tune_fit_rows != 2;
-        if (vmrc == 0 && !vm4)
-            vmrc = rch_fitvm_compile(prog + off[i], words, pcount[i], nv, VM_XS(2) * 4,
-                                     VMC_INPUTS_IN_U, vf.data(), (int)vf.size(), vr.data(),
-                                     (int)vr.size(), info);
-        if (vmrc == 0) {
+        const FtVmCode &vc = vmc[i];
+        const bool vm4 = vc.vm4;
+        if (vc.vmrc == 0) {
             if (code.size() & 1) code.push_back(0);
             d.vm = 1;
             d.fwd_off = (uint32_t)code.size();
-            code.insert(code.end(), vf.begin(), vf.begin() + 2 * info[0]);
+            code.insert(code.end(), vc.vf.begin(), vc.vf.end());
             d.rev_off = (uint32_t)code.size();
-            code.insert(code.end(), vr.begin(), vr.begin() + 2 * info[1]);
-            d.m = info[0];
-            d.nrev = info[1];
+            code.insert(code.end(), vc.vr.begin(), vc.vr.end());
+            d.m = (int)vc.vf.size() / 2;
+            d.nrev = (int)vc.vr.size() / 2;
         } else {
             d.fwd_off = (uint32_t)code.size();
             code.insert(code.end(), progs[i].fwd.begin(), progs[i].fwd.end());
@@ -1288,11 +1316,11 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
             d.nrev = (int)progs[i].rev.size();
         }
         int tcls;
-        if (vorc == 0) {
+        if (vc.vorc == 0) {
             if (code.size() & 1) code.push_back(0);
             d.t_off = (uint32_t)code.size();
-            code.insert(code.end(), vt.begin(), vt.begin() + 2 * tinfo[0]);
-            d.t_m = tinfo[0];
+            code.insert(code.end(), vc.vt.begin(), vc.vt.end());
+            d.t_m = (int)vc.vt.size() / 2;
             tcls = 0;
         } else {
             d.t_off = (uint32_t)code.size();
@@ -1321,14 +1349,24 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     if (code.empty()) code.push_back(0);
     const int nitems = (int)items.size();
     const int tri_stride = maxK * (maxK + 1) / 2;
-    /* Longest items first within each class list: fewer stragglers. */
-    auto work = [&](int a) {
-        return (long long)(cands[items[a].cand].m + cands[items[a].cand].nrev) * items[a].rows;
+    /* Longest items first within each class list: fewer stragglers.  The
+       sort runs on (work, item) pairs so the keys are read in place. */
+    std::vector<std::pair<long long, int>> keyed;
+    auto by_work = [&](std::vector<int> &l) {
+        keyed.resize(l.size());
+        for (size_t j = 0; j < l.size(); ++j) {
+            const FtItem &it = items[(size_t)l[j]];
+            const FtCand &d = cands[(size_t)it.cand];
+            keyed[j] = {(long long)(d.m + d.nrev) * it.rows, l[j]};
+        }
+        std::stable_sort(keyed.begin(), keyed.end(),
+                         [](const std::pair<long long, int> &a, const std::pair<long long, int> &b) {
+                             return a.first > b.first;
+                         });
+        for (size_t j = 0; j < l.size(); ++j) l[j] = keyed[j].second;
     };
-    for (int q = 0; q < NA; ++q)
-        std::stable_sort(list[q].begin(), list[q].end(), [&](int a, int b) { return work(a) > work(b); });
-    for (int q = 0; q < NT; ++q)
-        std::stable_sort(tlist[q].begin(), tlist[q].end(), [&](int a, int b) { return work(a) > work(b); });
+    for (int q = 0; q < NA; ++q) by_work(list[q]);
+    for (int q = 0; q < NT; ++q) by_work(tlist[q]);
     std::vector<FtRows> rowsd((size_t)c->nsets);
     for (int s = 0; s < c->nsets; ++s) {
         rowsd[s].cols = c->sets[s].cols;

@@ -355,7 +355,10 @@ int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int 
     C.vo = (flags & VMC_VALUE_ONLY) != 0;
     C.usink = (flags & VMC_SMALL_U) ? VM_U_SINK_SMALL : VM_U_SINK;
     C.nd.resize((size_t)m);
+    C.units.reserve((size_t)m);
+    C.spill.reserve((size_t)m);
     std::vector<int> st;
+    st.reserve((size_t)m);
     bool unsupported = false;
     for (int i = 0; i < m; ++i) {
         Nd &N = C.nd[(size_t)i];
@@ -420,6 +423,7 @@ int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int 
     /* Reverse stream: units in reverse order with explicit saved-adjoint
        slots. */
     std::vector<uint32_t> rv;
+    rv.reserve(C.vo ? 0 : 4 * C.units.size());
     int adepth = 0, apeak = 0;
     int popped = -1; /* the A slot the previous word ended by popping, else -1 */
     for (int i = C.vo ? -1 : (int)C.units.size() - 1; i >= 0; --i) {

@@ -16,6 +16,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <functional>
 #include <vector>
 
 #include "../../include/rcsr_hip.h"
@@ -167,6 +168,10 @@ int rc_dev_alloc(RcCtx *c, void **p, size_t bytes);
 void rc_dev_free(RcCtx *c, void *p);
 int rc_grow(RcCtx *c, void **p, size_t *cap, size_t need, size_t elem);
 int rc_rows_ensure_f64(RcCtx *c, RcRows *r);
+/* Run body(lo, hi) over [0, n) in contiguous chunks of at least min_chunk
+   indices on up to 8 host threads; the calling thread takes the first
+   chunk.  body may write only what belongs to its own indices. */
+void rc_parallel_for(int n, int min_chunk, const std::function<void(int, int)> &body);
 /* rc_order.hip: drop the cached target ranks (the f64 rows changed) / free
    the stage buffers. */
 int rc_order_invalidate(RcCtx *c);
