@@ -42,7 +42,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <unordered_map>
 
 #include <vector>
 
@@ -92,10 +91,10 @@ struct ENode {
    postfix order, so a subtree is the contiguous range [first, root].  Returns
    false for a program the reference device would score as NaN on every row;
    *root is the bottom entry of the stack at HALT, the value the reference
-   returns. */
+   returns.  The nodes are appended to `nodes` (operand indices are indices
+   into it); on failure the appended nodes are left for the caller to drop. */
 bool parse_one(const uint32_t *w, size_t len, int nvars, std::vector<ENode> &nodes, int *root)
 {
-    nodes.clear();
     int stk[RC_MAX_STACK];
     int depth = 0;
     size_t i = 0;
@@ -306,47 +305,174 @@ constexpr int RC_DERIVE_PASSES = 4;
 
 uint64_t derive_key(uint32_t op, int col) { return ((uint64_t)op << 32) | (uint32_t)col; }
 
-/* Mark the unary-of-column nodes of the live trees with their derived
-   columns, admitting pairs used at least twice (most used first) up to
-   max_derived columns in all.  derived holds (operator, source column) per
-   derived column; column nvars + k is pair k. */
-void assign_derived(Emitter &em, const std::vector<ENode> &all, const std::vector<int> &pstart,
-                    const std::vector<int> &proot, int nvars, std::vector<uint32_t> &derived,
-                    int max_derived)
-{
-    std::unordered_map<uint64_t, int> have;
-    for (size_t k = 0; k + 1 < derived.size(); k += 2)
-        have[derive_key(derived[k], (int)derived[k + 1])] = nvars + (int)(k / 2);
-    for (int pass = 0; pass < RC_DERIVE_PASSES; ++pass) {
-        std::unordered_map<uint64_t, int> count;
-        for (size_t p = 0; p + 1 < pstart.size(); ++p) {
-            if (proot[p] < 0) continue;
-            for (int j = pstart[p]; j <= proot[p]; ++j) {
+/* Open-addressing map from a nonzero derive key to an int (unary operator
+   codes are nonzero, so key 0 marks an empty slot). */
+struct KeyMap {
+    std::vector<uint64_t> key;
+    std::vector<int> val;
+    size_t mask = 0;
+
+    void reset(size_t n)
+    {
+        size_t cap = 16;
+        while (cap < 2 * n) cap <<= 1;
+        key.assign(cap, 0);
+        val.assign(cap, 0);
+        mask = cap - 1;
+    }
+    size_t slot(uint64_t k) const
+    {
+        size_t i = (size_t)((k * 0x9e3779b97f4a7c15ull) >> 17) & mask;
+        while (key[i] != 0 && key[i] != k) i = (i + 1) & mask;
+        return i;
+    }
+    const int *find(uint64_t k) const
+    {
+        const size_t i = slot(k);
+        return key[i] == k ? &val[i] : nullptr;
+    }
+    int &at(uint64_t k)
+    {
+        const size_t i = slot(k);
+        key[i] = k;
+        return val[i];
+    }
+};
+
+/* One contiguous range of a batch's programs: its parsed trees, their
+   analysis and its share of the streams.  Node and stream indices are local
+   to the chunk; the chunks are joined in program order. */
+struct Chunk {
+    int p0 = 0, p1 = 0;
+    std::vector<ENode> all;
+    std::vector<int> pstart, proot, depth;
+    std::vector<int> un;       /* unary nodes of the live trees, in tree order */
+    std::vector<uint32_t> end; /* end of each program's main stream in code */
+    Emitter em;
+    std::vector<uint32_t> code;
+    KeyMap count;
+    std::vector<uint64_t> seen;
+    int peak = 1;
+
+    void parse(const uint32_t *prog, const uint32_t *off, int nvars)
+    {
+        const int np = p1 - p0;
+        all.reserve(off[p1] > off[p0] ? (size_t)(off[p1] - off[p0]) : 0);
+        pstart.assign((size_t)np + 1, 0);
+        proot.assign((size_t)np, -1);
+        for (int q = 0; q < np; ++q) {
+            pstart[q] = (int)all.size();
+            const size_t a = off[p0 + q], b = off[p0 + q + 1];
+            int root = -1;
+            if (b > a && parse_one(prog + a, b - a, nvars, all, &root))
+                proot[q] = root;
+            else
+                all.resize((size_t)pstart[q]);
+        }
+        pstart[np] = (int)all.size();
+        em.code = &code;
+        em.analyse(all);
+        for (int q = 0; q < np; ++q) {
+            if (proot[q] < 0) continue;
+            for (int j = pstart[q]; j <= proot[q]; ++j) {
                 const ENode &n = all[j];
-                if (n.kid[0] < 0 || n.kid[1] >= 0 || n.op == RC_OP_NAN || em.colid[j] >= 0) continue;
-                const int src = em.colid[n.kid[0]];
-                if (src < 0) continue;
-                const uint64_t key = derive_key(n.op, src);
-                auto it = have.find(key);
-                if (it != have.end()) em.colid[j] = it->second;
-                else ++count[key];
+                if (n.kid[0] >= 0 && n.kid[1] < 0 && n.op != RC_OP_NAN) un.push_back(j);
             }
         }
-        if (pass == RC_DERIVE_PASSES - 1) break;
-        std::vector<std::pair<int, uint64_t>> cand;
-        for (const auto &kv : count)
-            if (kv.second >= 2) cand.push_back({-kv.second, kv.first});
-        std::sort(cand.begin(), cand.end());
-        int admitted = 0;
-        for (const auto &c : cand) {
-            if ((int)(derived.size() / 2) >= max_derived) break;
-            have[c.second] = nvars + (int)(derived.size() / 2);
-            derived.push_back((uint32_t)(c.second >> 32));
-            derived.push_back((uint32_t)(c.second & 0xffffffffu));
-            ++admitted;
-        }
-        if (admitted == 0) break;
     }
+
+    /* One pass of derived-column assignment: a unary-of-column node whose
+       pair is known reads that column (so its parent, later in the order,
+       is seen in the same pass); unknown pairs are counted when `tally`. */
+    void derive_pass(const KeyMap &have, bool tally)
+    {
+        if (tally) count.reset(un.size());
+        seen.clear();
+        for (const int j : un) {
+            if (em.colid[j] >= 0) continue;
+            const int src = em.colid[all[j].kid[0]];
+            if (src < 0) continue;
+            const uint64_t key = derive_key(all[j].op, src);
+            if (const int *col = have.find(key)) {
+                em.colid[j] = *col;
+            } else if (tally) {
+                int &c = count.at(key);
+                if (c++ == 0) seen.push_back(key);
+            }
+        }
+    }
+
+    void emit_all()
+    {
+        const int np = p1 - p0;
+        depth.assign((size_t)np, 1);
+        end.assign((size_t)np, 0);
+        for (int q = 0; q < np; ++q) {
+            const int root = proot[q];
+            int d;
+            if (root >= 0) {
+                em.depth = 0;
+                em.peak = 0;
+                if (em.height[root] <= RC_EMIT_MAX_HEIGHT) {
+                    em.emit(root);
+                } else {
+                    /* Plain postfix: the stack peak is the parse's. */
+                    em.plain(root, code);
+                    int dd = 0;
+                    for (int j = em.first[root]; j <= root; ++j) {
+                        int nk = 0;
+                        while (nk < 3 && all[j].kid[nk] >= 0) ++nk;
+                        dd += 1 - nk;
+                        em.peak = std::max(em.peak, dd);
+                    }
+                }
+                code.push_back(OP_HALT);
+                d = std::max(em.peak, 1);
+            } else {
+                code.push_back(OP_CONST);
+                code.push_back(0x7fc00000u);
+                code.push_back(OP_HALT);
+                d = 1;
+            }
+            depth[q] = d;
+            peak = std::max(peak, d);
+            end[q] = (uint32_t)code.size();
+        }
+    }
+};
+
+/* Admit the unknown pairs used at least twice across the chunks, most used
+   first, then by operator and column, up to max_derived columns in all.
+   Returns how many were admitted. */
+int admit_derived(std::vector<Chunk> &ch, KeyMap &have, int nvars, std::vector<uint32_t> &derived,
+                  int max_derived)
+{
+    size_t nseen = 0;
+    for (const Chunk &c : ch) nseen += c.seen.size();
+    KeyMap total;
+    total.reset(nseen);
+    std::vector<uint64_t> keys;
+    for (const Chunk &c : ch)
+        for (const uint64_t key : c.seen) {
+            int &t = total.at(key);
+            if (t == 0) keys.push_back(key);
+            t += *c.count.find(key);
+        }
+    std::vector<std::pair<int, uint64_t>> cand;
+    for (const uint64_t key : keys) {
+        const int t = *total.find(key);
+        if (t >= 2) cand.push_back({-t, key});
+    }
+    std::sort(cand.begin(), cand.end());
+    int admitted = 0;
+    for (const auto &c : cand) {
+        if ((int)(derived.size() / 2) >= max_derived) break;
+        have.at(c.second) = nvars + (int)(derived.size() / 2);
+        derived.push_back((uint32_t)(c.second >> 32));
+        derived.push_back((uint32_t)(c.second & 0xffffffffu));
+        ++admitted;
+    }
+    return admitted;
 }
 
 } /* namespace */
@@ -357,85 +483,85 @@ void assign_derived(Emitter &em, const std::vector<ENode> &all, const std::vecto
    cls_start receives the class boundaries in the program list.  derived
    (when given) holds the derived columns already computed, as (operator,
    source column) pairs for columns nvars, nvars + 1, ...; pairs the batch
-   admits are appended and must be computed before the streams run. */
+   admits are appended and must be computed before the streams run.
+
+   Derived columns: the unary-of-column nodes of the live trees are marked
+   with their columns in passes; between passes the pairs used at least
+   twice are admitted.  Programs are translated in chunks on host threads;
+   the output depends only on the batch, never on the chunking. */
 int rc_translate_programs(const uint32_t *prog, const uint32_t *off, int nprog, int nvars,
                           std::vector<uint32_t> &code, std::vector<uint32_t> &meta,
                           int cls_start[5], std::vector<uint32_t> *derived, int max_derived)
 {
     code.clear();
     meta.assign((size_t)nprog * 2, 0);
-    std::vector<ENode> all, one;
-    std::vector<int> pstart((size_t)nprog + 1, 0), proot((size_t)nprog, -1);
-    for (int p = 0; p < nprog; ++p) {
-        pstart[p] = (int)all.size();
-        const size_t a = off[p], b = off[p + 1];
-        int root = -1;
-        if (b > a && parse_one(prog + a, b - a, nvars, one, &root)) {
-            const int base = pstart[p];
-            for (ENode n : one) {
-                for (int k = 0; k < 3; ++k)
-                    if (n.kid[k] >= 0) n.kid[k] += base;
-                all.push_back(n);
-            }
-            proot[p] = base + root;
-        }
+    const int nch = std::max(1, std::min(8, nprog / 256));
+    std::vector<Chunk> ch((size_t)nch);
+    for (int i = 0; i < nch; ++i) {
+        ch[i].p0 = (int)((long long)nprog * i / nch);
+        ch[i].p1 = (int)((long long)nprog * (i + 1) / nch);
     }
-    pstart[nprog] = (int)all.size();
-    Emitter em;
-    em.code = &code;
-    em.analyse(all);
+    auto each = [&](const std::function<void(Chunk &)> &fn) {
+        rc_parallel_for(nch, 1, [&](int lo, int hi) {
+            for (int i = lo; i < hi; ++i) fn(ch[i]);
+        });
+    };
+    each([&](Chunk &c) { c.parse(prog, off, nvars); });
     if (derived) {
         const int cap = std::min(max_derived, 65535 - nvars);
-        assign_derived(em, all, pstart, proot, nvars, *derived, cap);
-    }
-    std::vector<int> depth((size_t)nprog, 1);
-    int peak = 1;
-    for (int p = 0; p < nprog; ++p) {
-        meta[p] = (uint32_t)code.size();
-        const int root = proot[p];
-        int d;
-        if (root >= 0) {
-            em.depth = 0;
-            em.peak = 0;
-            if (em.height[root] <= RC_EMIT_MAX_HEIGHT) {
-                em.emit(root);
-            } else {
-                /* Plain postfix: the stack peak is the parse's. */
-                em.plain(root, code);
-                int dd = 0;
-                for (int q = em.first[root]; q <= root; ++q) {
-                    int nk = 0;
-                    while (nk < 3 && all[q].kid[nk] >= 0) ++nk;
-                    dd += 1 - nk;
-                    em.peak = std::max(em.peak, dd);
-                }
+        size_t nun = 0;
+        for (const Chunk &c : ch) nun += c.un.size();
+        if (nun > 0) {
+            KeyMap have;
+            have.reset(derived->size() / 2 + std::min(nun, (size_t)std::max(0, cap)));
+            for (size_t k = 0; k + 1 < derived->size(); k += 2)
+                have.at(derive_key((*derived)[k], (int)(*derived)[k + 1])) = nvars + (int)(k / 2);
+            for (int pass = 0; pass < RC_DERIVE_PASSES; ++pass) {
+                const bool last = pass == RC_DERIVE_PASSES - 1;
+                each([&](Chunk &c) { c.derive_pass(have, !last); });
+                if (last || admit_derived(ch, have, nvars, *derived, cap) == 0) break;
             }
-            code.push_back(OP_HALT);
-            d = std::max(em.peak, 1);
-        } else {
-            code.push_back(OP_CONST);
-            code.push_back(0x7fc00000u);
-            code.push_back(OP_HALT);
-            d = 1;
         }
-        depth[p] = d;
-        peak = d > peak ? d : peak;
+    }
+    each([&](Chunk &c) { c.emit_all(); });
+    /* Join: main streams in program order, then the hoist section. */
+    size_t main_words = 0, hoist_words = 0, npatch = 0;
+    for (const Chunk &c : ch) {
+        main_words += c.code.size();
+        hoist_words += c.em.hcode.size();
+        npatch += c.em.patch.size();
+    }
+    code.reserve(main_words + hoist_words);
+    int peak = 1;
+    for (const Chunk &c : ch) {
+        const uint32_t base = (uint32_t)code.size();
+        for (int q = 0; q < c.p1 - c.p0; ++q)
+            meta[(size_t)(c.p0 + q)] = base + (q ? c.end[q - 1] : 0u);
+        code.insert(code.end(), c.code.begin(), c.code.end());
+        peak = std::max(peak, c.peak);
     }
     int k = 0;
     for (int cls = 0; cls < 4; ++cls) {
         cls_start[cls] = k;
-        for (int p = 0; p < nprog; ++p) {
-            const int d = depth[p];
-            const int mine = d <= 8 ? 0 : d <= 16 ? 1 : d <= 32 ? 2 : 3;
-            if (mine == cls) meta[(size_t)nprog + k++] = (uint32_t)p;
-        }
+        for (const Chunk &c : ch)
+            for (int q = 0; q < c.p1 - c.p0; ++q) {
+                const int d = c.depth[q];
+                const int mine = d <= 8 ? 0 : d <= 16 ? 1 : d <= 32 ? 2 : 3;
+                if (mine == cls) meta[(size_t)nprog + k++] = (uint32_t)(c.p0 + q);
+            }
     }
     cls_start[4] = k;
     const uint32_t hbase = (uint32_t)code.size();
-    code.insert(code.end(), em.hcode.begin(), em.hcode.end());
-    for (size_t q = 0; q < em.patch.size(); q += 2) {
-        meta.push_back(hbase + em.patch[q]);
-        meta.push_back(em.patch[q + 1]);
+    meta.reserve(meta.size() + npatch);
+    uint32_t mbase = 0, hofs = 0;
+    for (const Chunk &c : ch) {
+        code.insert(code.end(), c.em.hcode.begin(), c.em.hcode.end());
+        for (size_t q = 0; q < c.em.patch.size(); q += 2) {
+            meta.push_back(hbase + hofs + c.em.patch[q]);
+            meta.push_back(mbase + c.em.patch[q + 1]);
+        }
+        mbase += (uint32_t)c.code.size();
+        hofs += (uint32_t)c.em.hcode.size();
     }
     return peak;
 }
