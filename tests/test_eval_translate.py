@@ -263,6 +263,46 @@ def test_random_programs_translate_to_the_same_values():
     assert nfused > 100 and len(meta) > 2 * len(progs)  # fused words and hoists both occur
 
 
+def test_large_batches_translate_to_the_same_values():
+    # 1100 programs: the batch is translated in several program chunks whose
+    # streams, hoist programs and derived-column counts are joined
+    rng = np.random.default_rng(15)
+    X = _x(rng, 32)
+    progs = [random_program(rng, 3, int(rng.integers(1, 40))) for _ in range(1100)]
+    code, meta, info, pairs = check_batch(progs, 3, X)
+    assert len(pairs) > 0 and len(meta) > 2 * len(progs)
+
+
+def test_batch_streams_equal_single_program_streams():
+    # Without derived columns a program's stream and hoist programs depend
+    # on the program alone: the joined batch holds each single translation.
+    rng = np.random.default_rng(16)
+    progs = [random_program(rng, 3, int(rng.integers(1, 40))) for _ in range(1300)]
+    prog, off = pack(progs)
+    code, meta, info, _ = translate(prog, off, len(progs), 3, None)
+    n = len(progs)
+    ends = list(meta[1:n]) + [int(meta[2 * n]) if len(meta) > 2 * n else len(code)]
+    hoists = {}
+    for hp, target in meta[2 * n:].reshape(-1, 2):
+        q = int(hp)
+        while code[q] != HALT:
+            q += 1
+        hoists[int(target)] = code[int(hp):q + 1].tolist()
+    for p in range(0, n, 7):
+        sp, so = pack([progs[p]])
+        c1, m1, _, _ = translate(sp, so, 1, 3, None)
+        main1 = int(m1[2]) if len(m1) > 2 else len(c1)
+        assert code[meta[p]:ends[p]].tolist() == c1[:main1].tolist(), p
+        h1 = {}
+        for hp, target in m1[2:].reshape(-1, 2):
+            q = int(hp)
+            while c1[q] != HALT:
+                q += 1
+            h1[int(target)] = c1[int(hp):q + 1].tolist()
+        mine = {t - int(meta[p]): h for t, h in hoists.items() if meta[p] <= t < ends[p]}
+        assert mine == h1, p
+
+
 def test_c2_programs_fuse_and_hoist():
     wl = workloads.config2(n=64, nprog=256)
     pop = host.Population()
