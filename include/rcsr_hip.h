@@ -89,9 +89,9 @@ int rc_set_fit_data(void *ctx, const double *cols, const double *y, const double
    NULL clears them. */
 int rc_set_score_weights(void *ctx, const double *w);
 
-/* eval.cu:3955 rc_set_fit_sign: per-row label sign t = 2l - 1 for the
-   logistic arm; NULL clears it.  The logistic fit objective itself is not
-   built here: a fit with metric 5 returns -1 (the caller keeps its CPU arm). */
+/* eval.cu:3955 rc_set_fit_sign: per-row label sign t = 2l - 1 of the
+   resident rows for the logistic fit arm (metric 5) and the classification
+   scores (10-14); NULL clears it, after which those launches return -1. */
 int rc_set_fit_sign(void *ctx, const double *sign);
 
 /* ---- evaluation -------------------------------------------------------- */
@@ -139,6 +139,8 @@ int rc_pool_hash(void *ctx, int which, int nprog, int width, unsigned long long 
    c0 nprog x kmax start constants; gauge (may be NULL) holds per candidate
    the (scale, offset) constant slots of its outer affine pair, -1 for none.
    metric: 0 squared, 1 log-squashed, 4 absolute (objective sum w*rho(r));
+   5 logistic (eval.cu:683-705, 1569-1576: objective sum w ln(1 + exp(-t p))
+   with the signs of rc_set_fit_sign, Fisher-weighted LM; -1 without signs);
    2 correlation 1 - |rho|, 3 signed correlation 1 - rho (eval.cu:2320-2448:
    the gauge pair is renormalised to (sign(a), 0) and frozen; objective the
    weighted Pearson form).  Outputs: constants (nprog x kmax), the fitted
@@ -222,7 +224,7 @@ int rc_add_rows(void *ctx, const float *cols, int n, int v, const float *y,
                 const double *cols64, const double *y64, const double *w);
 
 /* NEW: heterogeneous batch in one launch sequence: per-candidate row set,
-   metric (0 / 1 / 4) and the shared iteration count.  Same outputs as
+   metric (0 / 1 / 4, or 5 on row set 0) and the shared iteration count.  Same outputs as
    rc_fit_batch. */
 float rc_fit_batch_hetero(void *ctx, const unsigned int *prog, const unsigned int *off,
                           const int *pcount, int nprog, int kmax, const double *c0,

@@ -718,12 +718,45 @@ std::vector<double> predictions(const Node &tree, const Data &D)
     return p;
 }
 
+/* ---- metrics.rs:1467-1475 binary_labels: label 1 above the midpoint of
+   the target's range, else 0. */
+std::vector<double> binary_labels(const Data &D)
+{
+    double lo = INF, hi = -INF;
+    for (int i = 0; i < D.n; ++i) {
+        lo = std::fmin(lo, D.y[i]);
+        hi = std::fmax(hi, D.y[i]);
+    }
+    const double mid = 0.5 * (lo + hi);
+    std::vector<double> l(D.n);
+    for (int i = 0; i < D.n; ++i) l[i] = D.y[i] > mid ? 1.0 : 0.0;
+    return l;
+}
+
+/* ---- metrics.rs:1547-1550 softplus_neg: ln(1 + exp(-u)) without overflow. */
+double softplus_neg(double u)
+{
+    const double a = std::fabs(u);
+    return std::fmax(-u, 0.0) + std::log1p(std::exp(-a));
+}
+
+/* ---- metrics.rs:1610-1620 log_loss (unweighted): mean softplus_neg(t p). */
+double log_loss_pred(const std::vector<double> &pred, const Data &D)
+{
+    if (pred.empty()) return INF;
+    const std::vector<double> l = binary_labels(D);
+    double acc = 0.0;
+    for (size_t i = 0; i < pred.size(); ++i) acc += softplus_neg((2.0 * l[i] - 1.0) * pred[i]);
+    return acc / (double)pred.size();
+}
+
 /* ---- metrics.rs:852-900 (unweighted) and 985-1060 (weighted) for the
    squared / log-squashed / absolute rows. */
 double score_pred(const std::vector<double> &pred, const Data &D, int metric)
 {
     const int n = (int)pred.size();
     if (n == 0) return INF;
+    if (metric == OR_METRIC_LOG_LOSS) return log_loss_pred(pred, D);
     auto rho = [&](double e) {
         if (metric == OR_METRIC_ABSOLUTE) return std::fabs(e);
         if (metric == OR_METRIC_LOG_SQUASHED) return std::log(1.0 + e * e);
@@ -815,9 +848,60 @@ double irls_fit(Node &tree, const Data &D, int iters, int metric)
     return best;
 }
 
-/* ---- fit.rs:2937-2975: fit_weighted_metric (squared / log / absolute). */
+/* ---- fit.rs:3271-3331: logistic_irls.  Each round solves the Newton
+   step's weighted least squares min sum w q (1 - q) (p - z)^2 with
+   q = sigmoid(t p) clamped to [1e-9, 1 - 1e-9] and z = p + t / q clamped to
+   [-1e6, 1e6] (two inner fit iterations), and keeps the trial only when the
+   log loss strictly improves; at most min(max(iters, 1), 16) rounds. */
+double logistic_irls(Node &tree, const Data &D, int iters)
+{
+    double best = metric_score(tree, D, OR_METRIC_LOG_LOSS);
+    if (!std::isfinite(best)) return best;
+    const std::vector<double> l = binary_labels(D);
+    const int rounds = std::min(std::max(iters, 1), 16);
+    std::vector<double> target(D.n), weights(D.n);
+    for (int round = 0; round < rounds; ++round) {
+        const std::vector<double> pred = predictions(tree, D);
+        bool finite = true;
+        for (double p : pred)
+            if (!std::isfinite(p)) { finite = false; break; }
+        if (!finite) break;
+        for (int i = 0; i < D.n; ++i) {
+            const double t = 2.0 * l[i] - 1.0;
+            const double u = t * pred[i];
+            double q;
+            if (u >= 0.0) {
+                q = 1.0 / (1.0 + std::exp(-u));
+            } else {
+                const double e = std::exp(u);
+                q = e / (1.0 + e);
+            }
+            q = std::fmin(std::fmax(q, 1e-9), 1.0 - 1e-9);
+            const double base = D.w ? D.w[i] : 1.0;
+            weights[i] = base > 0.0 ? base * q * (1.0 - q) : 0.0;
+            target[i] = std::fmin(std::fmax(pred[i] + t / q, -1e6), 1e6);
+        }
+        Node trial = tree;
+        Data Dw = D;
+        Dw.y = target.data();
+        Dw.w = weights.data();
+        fit_weighted(trial, Dw, 2);
+        const double score = metric_score(trial, D, OR_METRIC_LOG_LOSS);
+        if (std::isfinite(score) && score < best) {
+            best = score;
+            tree = trial;
+        } else {
+            break;
+        }
+    }
+    return best;
+}
+
+/* ---- fit.rs:2937-2990: fit_weighted_metric (squared / log / absolute /
+   log loss). */
 double fit_metric(Node &tree, const Data &D, int iters, int metric)
 {
+    if (metric == OR_METRIC_LOG_LOSS) return logistic_irls(tree, D, iters);
     if (iters == 0) {
         if (metric == OR_METRIC_SQUARED) return fit_weighted(tree, D, 0);
         return metric_score(tree, D, metric);

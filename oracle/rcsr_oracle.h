@@ -39,6 +39,9 @@ enum {
 
 /* Metric codes (the device fit codes): squared, log-squashed, absolute. */
 enum { OR_METRIC_SQUARED = 0, OR_METRIC_LOG_SQUASHED = 1, OR_METRIC_ABSOLUTE = 4 };
+/* The log-loss metric (the logistic arm's objective) of or_fit_metric /
+   or_metric_score: mean ln(1 + exp(-t p)) with t from the target midpoint. */
+enum { OR_METRIC_LOG_LOSS = 10 };
 
 /* Arity of an operator code, -1 for an unknown code. */
 int or_arity(int op);

@@ -213,6 +213,72 @@ __device__ __forceinline__ double rc_row_weight(double base, double r, int metri
     return base;
 }
 
+/* ---- The logistic fit arm (eval.cu:683-705, 1569-1576).  The row's target
+   slot holds the label sign t = +-1; the margin is u = t p. */
+/* sigmoid(u) in the branch form that cannot overflow, clamped to
+   [1e-9, 1 - 1e-9] so t / q and q (1 - q) stay finite. */
+__device__ __forceinline__ double rc_logistic_q(double u)
+{
+    double q;
+    if (u >= 0.0) {
+        q = 1.0 / (1.0 + exp(-u));
+    } else {
+        const double e = exp(u);
+        q = e / (1.0 + e);
+    }
+    return fmin(1.0 - 1e-9, fmax(1e-9, q));
+}
+
+/* ln(1 + exp(-u)) as max(-u, 0) + log1p(exp(-|u|)): no overflow, no
+   cancellation for small |u|. */
+__device__ __forceinline__ double rc_softplus_neg(double u)
+{
+    const double a = fabs(u);
+    return (u < 0.0 ? -u : 0.0) + log1p(exp(-a));
+}
+
+/* The logistic row terms are out of line: the branch is wave-uniform and
+   rarely taken, and inlined f64 exp / log1p would cost the other metrics'
+   kernels registers. */
+__device__ __noinline__ double rc_softplus_neg_call(double u) { return rc_softplus_neg(u); }
+
+/* The row loss of prediction p against the target slot y. */
+__device__ __forceinline__ double rc_row_stat(double p, double y, int metric)
+{
+    if (metric == RC_LOSS_LOGISTIC) return rc_softplus_neg_call(y * p);
+    return rc_row_rho(p - y, metric);
+}
+
+/* The row of the weighted least-squares step: s = sqrt(row weight), the
+   residual r the row carries and the row loss rho.  Logistic: the Fisher
+   weight w q (1 - q) with q = sigmoid(t p) and the residual -t / q, so that
+   sum (s g)(s r) = -sum w t (1 - q) g is the loss gradient. */
+struct RcRowFit {
+    double s, r, rho;
+};
+__device__ __noinline__ RcRowFit rc_row_fit_logistic(double wb, double p, double y)
+{
+    RcRowFit o;
+    const double u = y * p;
+    const double q = rc_logistic_q(u);
+    o.s = sqrt(wb * q * (1.0 - q));
+    o.r = -y / q;
+    o.rho = rc_softplus_neg(u);
+    return o;
+}
+__device__ __forceinline__ RcRowFit rc_row_fit(double wb, double p, double y, int metric)
+{
+    RcRowFit o;
+    if (metric == RC_LOSS_LOGISTIC) {
+        o = rc_row_fit_logistic(wb, p, y);
+    } else {
+        o.r = p - y;
+        o.s = sqrt(rc_row_weight(wb, o.r, metric));
+        o.rho = rc_row_rho(o.r, metric);
+    }
+    return o;
+}
+
 /* ---- Wave-level helpers (wave64). */
 __device__ __forceinline__ double rc_wave_sum_f64(double v)
 {

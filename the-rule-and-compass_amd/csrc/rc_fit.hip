@@ -396,19 +396,21 @@ __global__ __launch_bounds__(64) void ft_asm_kernel(
             p64 = ft_forward<double, BIG, false>(fc, d.m, cp64 + (size_t)cand * FT_KMAX, xd,
                                                  R.cols64, R.n, rr, none);
         }
-        const double r64 = p64 - R.y64[rr];
-        const bool ok = live && isfinite(p64) && (polish || fabsf(pred) <= 1e30f) && isfinite(r64);
-        const double s64 = sqrt(rc_row_weight(wb, r64, d.metric));
+        const double y64 = R.y64[rr];
+        const bool ok = live && isfinite(p64) && (polish || fabsf(pred) <= 1e30f) &&
+                        isfinite(p64 - y64);
+        const RcRowFit rf = rc_row_fit(wb, p64, y64, d.metric);
+        const double r64 = rf.r, s64 = rf.s;
         const float seed = ok ? (float)s64 : 0.0f;
         const bool gfin = ft_reverse<float, BIG>(rc, d.nrev, seed, tape, X, lane, (float *)nullptr);
         const bool use = ok && gfin;
         if (polish) {
             if (live && !ok) bad = 1;
             if (ok && !gfin) ++skip;
-            if (ok) lsum += wb * rc_row_rho(r64, d.metric);
+            if (ok) lsum += wb * rf.rho;
         } else {
             if (live && !use) bad = 1;
-            if (use) lsum += wb * rc_row_rho(r64, d.metric);
+            if (use) lsum += wb * rf.rho;
         }
         if (__any(ok && !gfin)) {
             if (!use)
@@ -524,10 +526,11 @@ __global__ __launch_bounds__(64) void ft_trial_kernel(
         for (int r = 0; r < R; ++r) {
             const double wb = R_.w64 ? R_.w64[xrow[r]] : 1.0;
             if (!live[r] || !(wb > 0.0)) continue;
-            const double rr = (double)pred[r] - R_.y64[xrow[r]];
+            const double yr = R_.y64[xrow[r]];
+            const double rr = (double)pred[r] - yr;
             bool hard = !isfinite(pred[r]) || !isfinite(rr);
             if (sizeof(T) == 4) hard = hard || fabs((double)pred[r]) > 1e30;
-            lsum += hard ? 1e30 : wb * rc_row_rho(rr, d.metric);
+            lsum += hard ? 1e30 : wb * rc_row_stat((double)pred[r], yr, d.metric);
         }
     }
     const double ls = rc_wave_sum_f64(lsum);
@@ -1185,6 +1188,25 @@ static bool prog_is_big(const FtProg &p)
            p.tape_r + 1 > SMALL_TAPE;
 }
 
+/* The row sets as the fit kernels see them: one per resident set, then the
+   resident rows with the label signs in the target slot (the logistic arm's
+   set, index nsets; its target is null without rc_set_fit_sign). */
+std::vector<FtRows> rc_fit_rows(RcCtx *c)
+{
+    std::vector<FtRows> rowsd((size_t)c->nsets + 1);
+    for (int s = 0; s < c->nsets; ++s) {
+        rowsd[s].cols = c->sets[s].cols;
+        rowsd[s].cols64 = c->sets[s].cols64;
+        rowsd[s].y64 = c->sets[s].y64;
+        rowsd[s].w64 = c->sets[s].w64;
+        rowsd[s].n = c->sets[s].n;
+        rowsd[s].v = c->sets[s].v;
+    }
+    rowsd[c->nsets] = rowsd[0];
+    rowsd[c->nsets].y64 = c->d_tsign64;
+    return rowsd;
+}
+
 /* One candidate's fit-VM streams (word pairs, trimmed to length). */
 struct FtVmCode {
     int vorc = -1, vmrc = -1;
@@ -1237,7 +1259,12 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         const int rs = rowset_in ? rowset_in[i] : 0;
         const int mt = metric_in ? metric_in[i] : metric_all;
         if (rs < 0 || rs >= c->nsets) return -1.0f;
-        if (mt != RC_LOSS_SQUARED && mt != RC_LOSS_LOG_SQUASHED && mt != RC_LOSS_ABSOLUTE) return -1.0f;
+        if (mt != RC_LOSS_SQUARED && mt != RC_LOSS_LOG_SQUASHED && mt != RC_LOSS_ABSOLUTE &&
+            mt != RC_LOSS_LOGISTIC)
+            return -1.0f;
+        /* The logistic arm needs the label signs of the resident rows
+           (rc_set_fit_sign), like the classification scores. */
+        if (mt == RC_LOSS_LOGISTIC && (rs != 0 || !c->has_tsign || !c->d_tsign64)) return -1.0f;
         rowset[i] = rs;
         metric[i] = mt;
         if (mt == RC_LOSS_ABSOLUTE) any_irls = true;
@@ -1290,7 +1317,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     for (int i = 0; i < nprog; ++i) {
         FtCand &d = cands[i];
         memset(&d, 0, sizeof(d));
-        d.rowset = rowset[i];
+        d.rowset = metric[i] == RC_LOSS_LOGISTIC ? c->nsets : rowset[i];
         d.metric = metric[i];
         pre[i] = progs[i].pre_status;
         /* A host-decided candidate keeps k = 0: its outputs are c0 verbatim. */
@@ -1328,7 +1355,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
             d.t_m = (int)progs[i].fwd.size();
             tcls = progs[i].max_stack > SMALL_S ? 2 : 1;
         }
-        const int n = c->sets[d.rowset].n;
+        const int n = c->sets[rowset[i]].n;
         d.item0 = (int)items.size();
         const bool big = prog_is_big(progs[i]);
         const int kc = d.k + 2 <= 16 ? 0 : 1;
@@ -1367,15 +1394,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     };
     for (int q = 0; q < NA; ++q) by_work(list[q]);
     for (int q = 0; q < NT; ++q) by_work(tlist[q]);
-    std::vector<FtRows> rowsd((size_t)c->nsets);
-    for (int s = 0; s < c->nsets; ++s) {
-        rowsd[s].cols = c->sets[s].cols;
-        rowsd[s].cols64 = c->sets[s].cols64;
-        rowsd[s].y64 = c->sets[s].y64;
-        rowsd[s].w64 = c->sets[s].w64;
-        rowsd[s].n = c->sets[s].n;
-        rowsd[s].v = c->sets[s].v;
-    }
+    const std::vector<FtRows> rowsd = rc_fit_rows(c);
     std::vector<int> flat_lists;
     int lofs[NA + 1], tofs[NT + 1];
     for (int q = 0; q < NA; ++q) {

@@ -106,16 +106,17 @@ __device__ void p64_assemble(P64Smem &S, const uint32_t *nd, int m, const FtRows
                 double v[FT_MAX_NODES], adj[FT_MAX_NODES], g[FT_KMAX];
                 p64_forward(nd, m, R, row, c, v);
                 const double pred = v[m - 1];
-                const double r = pred - R.y64[row];
-                const bool ok = isfinite(pred) && isfinite(r);
+                const double yv = R.y64[row];
+                const bool ok = isfinite(pred) && isfinite(pred - yv);
                 p64_backward(nd, m, v, adj, g, p);
                 bool gok = true;
                 for (int j = 0; j < p; ++j) if (!isfinite(g[j])) gok = false;
-                const double s = sqrt(rc_row_weight(wr, r, metric));
+                const RcRowFit rf = rc_row_fit(wr, pred, yv, metric);
+                const double r = rf.r, s = rf.s;
                 const bool use = ok && gok;
                 for (int j = 0; j < p; ++j) dgl[j] = use ? s * g[j] : 0.0;
                 dr = use ? s * r : 0.0;
-                dl = (polish ? ok : use) ? wr * rc_row_rho(r, metric) : 0.0;
+                dl = (polish ? ok : use) ? wr * rf.rho : 0.0;
                 if (polish) {
                     if (!ok) S.sbad = 1;
                     if (ok && !gok) atomicAdd(&S.sskip, 1);
@@ -160,8 +161,8 @@ __device__ double p64_loss(P64Smem &S, const uint32_t *nd, int m, const FtRows &
         double v[FT_MAX_NODES];
         p64_forward(nd, m, R, row, c, v);
         const double pred = v[m - 1];
-        const double r = pred - R.y64[row];
-        local += (!isfinite(pred) || !isfinite(r)) ? 1e30 : wr * rc_row_rho(r, metric);
+        const double yv = R.y64[row];
+        local += (!isfinite(pred) || !isfinite(pred - yv)) ? 1e30 : wr * rc_row_stat(pred, yv, metric);
     }
     S.sred[tid] = local;
     __syncthreads();
@@ -316,19 +317,12 @@ int rc_fit64_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const int 
         cands[i].off = off[i];
         cands[i].m = (int)(off[i + 1] - off[i]) / 4;
         cands[i].k = pre[i] >= 0 ? 0 : pcount[i];
-        cands[i].rowset = rowset[i];
+        /* the logistic arm reads the label signs in the target slot */
+        cands[i].rowset = metric[i] == RC_LOSS_LOGISTIC ? c->nsets : rowset[i];
         cands[i].metric = metric[i];
         cands[i].pre = pre[i];
     }
-    std::vector<FtRows> rowsd((size_t)c->nsets);
-    for (int s = 0; s < c->nsets; ++s) {
-        rowsd[s].cols = c->sets[s].cols;
-        rowsd[s].cols64 = c->sets[s].cols64;
-        rowsd[s].y64 = c->sets[s].y64;
-        rowsd[s].w64 = c->sets[s].w64;
-        rowsd[s].n = c->sets[s].n;
-        rowsd[s].v = c->sets[s].v;
-    }
+    const std::vector<FtRows> rowsd = rc_fit_rows(c);
     const size_t words = off[nprog];
     uint32_t *dp = nullptr;
     P64Cand *dc = nullptr;

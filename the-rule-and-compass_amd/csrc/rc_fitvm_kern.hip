@@ -20,7 +20,7 @@ namespace {
 
 /* Per-row residual bookkeeping shared by the assembly kernels. */
 struct RowPrep {
-    double r64, s64, wb;
+    double r64, s64, rho, wb;
     bool live, ok;
 };
 
@@ -33,9 +33,18 @@ __device__ __forceinline__ RowPrep vm_row_prep(bool live, double wb, float pred,
     RowPrep q;
     q.live = live;
     q.wb = wb;
-    q.r64 = p64 - y64;
-    q.ok = live && isfinite(p64) && (polish || fabsf(pred) <= 1e30f) && isfinite(q.r64);
-    q.s64 = UNIT ? 1.0 : sqrt(rc_row_weight(wb, q.r64, metric));
+    const double d64 = p64 - y64;
+    q.ok = live && isfinite(p64) && (polish || fabsf(pred) <= 1e30f) && isfinite(d64);
+    if (UNIT) {
+        q.r64 = d64;
+        q.s64 = 1.0;
+        q.rho = rc_row_rho(d64, RC_LOSS_SQUARED);
+    } else {
+        const RcRowFit f = rc_row_fit(wb, p64, y64, metric);
+        q.r64 = f.r;
+        q.s64 = f.s;
+        q.rho = f.rho;
+    }
     return q;
 }
 
@@ -190,10 +199,10 @@ __global__ __launch_bounds__(64) void vm_asm_kernel(
             if (POLISH) {
                 if (a[r].live && !a[r].ok) bad = 1;
                 skip += (a[r].ok && !gf) ? 1 : 0;
-                if (a[r].ok) lsum += a[r].wb * rc_row_rho(a[r].r64, d.metric);
+                if (a[r].ok) lsum += a[r].wb * a[r].rho;
             } else {
                 if (a[r].live && !use[r]) bad = 1;
-                if (use[r]) lsum += a[r].wb * rc_row_rho(a[r].r64, d.metric);
+                if (use[r]) lsum += a[r].wb * a[r].rho;
             }
             fix = fix || (a[r].ok && !gf);
         }
@@ -366,7 +375,7 @@ __global__ __launch_bounds__(64) void vm_trial_kernel(
             const double rr = (double)pred[r] - yv[r];
             bool hard = !isfinite(pred[r]) || !isfinite(rr);
             if (sizeof(T) == 4) hard = hard || fabs((double)pred[r]) > 1e30;
-            const double term = hard ? 1e30 : wb * rc_row_rho(rr, d.metric);
+            const double term = hard ? 1e30 : wb * rc_row_stat((double)pred[r], yv[r], d.metric);
             lsum += (live[r] && wb > 0.0) ? term : 0.0;
         }
     }

@@ -168,6 +168,10 @@ int rc_dev_alloc(RcCtx *c, void **p, size_t bytes);
 void rc_dev_free(RcCtx *c, void *p);
 int rc_grow(RcCtx *c, void **p, size_t *cap, size_t need, size_t elem);
 int rc_rows_ensure_f64(RcCtx *c, RcRows *r);
+struct FtRows;
+/* rc_fit.hip: the fit kernels' row-set table (resident sets, then the
+   logistic arm's view of set 0 with the label signs as its target). */
+std::vector<FtRows> rc_fit_rows(RcCtx *c);
 /* Run body(lo, hi) over [0, n) in contiguous chunks of at least min_chunk
    indices on up to 8 host threads; the calling thread takes the first
    chunk.  body may write only what belongs to its own indices. */
