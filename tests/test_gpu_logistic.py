@@ -168,3 +168,30 @@ def test_the_launch_is_refused_without_label_signs():
         assert ms >= 0
     finally:
         ev.close()
+
+
+def test_plain_fp64_arm_and_mixed_batches_fit_the_same_objective():
+    x, y = _sample(1200, 11)
+    start = _affine(0.2, 0.1)
+    ev = _evaluator(x, y)
+    try:
+        mixed, ms0 = ev.fit_batch([host.gpu_fit_plan(start)], 12, LOGISTIC)
+        het, ms1 = ev.fit_batch_hetero([host.gpu_fit_plan(start)] * 3, [0, 0, 0],
+                                       [LOGISTIC, 0, LOGISTIC], 12)
+        assert ev.set_fit_mode(1) == 0          # returns the previous mode
+        plain, ms2 = ev.fit_batch([host.gpu_fit_plan(start)], 12, LOGISTIC)
+        assert ev.set_fit_mode(0) == 1
+    finally:
+        ev.close()
+    assert min(ms0, ms1, ms2) >= 0
+    # a candidate's result does not depend on its neighbours' metrics
+    assert het[0].sse == mixed[0].sse and np.array_equal(het[0].c, mixed[0].c)
+    assert het[2].sse == mixed[0].sse
+    # the squared neighbour reports its own objective
+    assert het[1].sse == pytest.approx(
+        float(np.sum((oracle.predict(start.with_constants(het[1].c), x) - y) ** 2)), rel=1e-10)
+    for o in (mixed[0], plain[0]):
+        assert host.fit_status_usable(o.status), o.status
+        assert o.sse == pytest.approx(
+            _softplus_sum(oracle.predict(start.with_constants(o.c), x), y), rel=1e-12)
+    assert abs(plain[0].sse - mixed[0].sse) / mixed[0].sse <= 1e-6
