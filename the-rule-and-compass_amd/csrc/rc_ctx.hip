@@ -38,8 +38,14 @@ void rc_parallel_for(int n, int min_chunk, const std::function<void(int, int)> &
     }
     std::vector<std::thread> pool;
     pool.reserve((size_t)nt - 1);
-    for (int t = 1; t < nt; ++t)
-        pool.emplace_back(body, (int)((long long)n * t / nt), (int)((long long)n * (t + 1) / nt));
+    for (int t = 1; t < nt; ++t) {
+        const int lo = (int)((long long)n * t / nt), hi = (int)((long long)n * (t + 1) / nt);
+        try {
+            pool.emplace_back(body, lo, hi);
+        } catch (...) {
+            body(lo, hi); /* no thread available: run the chunk here */
+        }
+    }
     body(0, (int)((long long)n / nt));
     for (std::thread &th : pool) th.join();
 }
