@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--eval-progs", type=int, default=4096)
     ap.add_argument("--no-c4", action="store_true",
                     help="skip the uneven-batch (autotuned vs fixed shapes) part")
+    ap.add_argument("--no-arms", action="store_true",
+                    help="skip the logistic / correlation fit arm lines")
     ap.add_argument("--no-order", action="store_true",
                     help="skip the exact order / rank / maximum stages part")
     ap.add_argument("--order-only", action="store_true",
@@ -276,7 +278,41 @@ def main():
     # Exact order / rank / maximum stages on the C3 rows.
     if not args.no_order:
         result["order_stages"] = order_part(host, workloads)
+    # The logistic and correlation fit arms on a sample of the C3 batch.
+    if not args.no_arms:
+        result["other_fit_arms"] = arms_part(host, workloads)
     print(json.dumps(result), flush=True)
+
+
+def arms_part(host, workloads, ncand=256):
+    """The logistic (metric 5, labels y > midpoint of the C3 target) and
+    correlation (metric 2) arms: the first 256 C3 candidates, N = 262144,
+    30 LM iterations, inputs resident; candidates/s over the best of 2."""
+    import numpy as np
+    wl = workloads.config3(n=262144, ncand=ncand, iters=30)
+    plans = [host.gpu_fit_plan(t) for t in wl.trees]
+    ev = host.GpuEvaluator(wl.cols32(), wl.n, wl.v, wl.y.astype(np.float32))
+    assert ev.set_fit_data(wl.cols64(), wl.y)
+    mid = 0.5 * (wl.y.min() + wl.y.max())
+    assert ev.set_fit_sign(np.where(wl.y > mid, 1.0, -1.0))
+    out = {"workload": f"first {ncand} C3 candidates, N={wl.n}, 30 LM iterations"}
+    for name, metric in (("logistic", 5), ("correlation", 2)):
+        outs, ms = ev.fit_batch(plans, 30, metric, kmax=20)
+        walls = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            outs, ms = ev.fit_batch(plans, 30, metric, kmax=20)
+            walls.append(time.perf_counter() - t0)
+        if ms < 0:
+            raise RuntimeError(f"{name} arm failed")
+        st = {}
+        for o in outs:
+            st[str(o.status)] = st.get(str(o.status), 0) + 1
+        out[name] = {"candidates_per_s": ncand / min(walls), "ms_wall": 1e3 * min(walls),
+                     "status_counts": st}
+        log(f"{name} arm: {ncand / min(walls):.0f} candidates/s ({1e3 * min(walls):.1f} ms)")
+    ev.close()
+    return out
 
 
 def order_part(host, workloads, ncand=64, ntrials=4):
