@@ -57,6 +57,17 @@ int rch_eval_translate(const unsigned int *prog, const unsigned int *off, int np
                        unsigned int *code, int cap_code, unsigned int *meta, int cap_meta,
                        unsigned int *derived, int cap_derived, int *nderived, int *info);
 
+/* No reference counterpart: the replay key of a recorded fit graph for a
+   region of the given sizes (code words, candidate records, row-set records,
+   work items, class-list words, candidates, kmax, Gram triangle stride),
+   with a null region base, 30 LM slots and empty class lists.  The key holds
+   every region offset, so two batches whose regions end at the same byte
+   but place a table elsewhere never replay each other's graph.  Exported so
+   the key can be checked on the CPU.  Writes up to cap words to key.
+   Returns the word count, -1 bad arguments, -2 cap too small. */
+int rch_fit_graph_key(int code_words, int ncands, int nrowsets, int nitems, int nlist_words,
+                      int nprog, int kmax, int tri_stride, unsigned long long *key, int cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -446,3 +446,44 @@ def test_many_inputs_and_shared_subtrees_match_the_oracle():
         fin = ~np.isnan(q)
         assert np.array_equal(q[fin].view(np.uint32), pred[i][fin].view(np.uint32)), str(trees[i])
     ev.close()
+
+
+def test_pipelined_submit_upload_collect_keeps_each_batch_timing(data):
+    # The reference's pipelined sequence (gpu.rs:1463-1490): upload A, submit
+    # A, upload B while A runs, collect A, submit B, collect B.  Each batch's
+    # time is non-negative and its losses equal a plain rc_eval of it; a
+    # score call between the upload of B and its submit keeps its own events.
+    from rcsr_hip import lib as L
+    x, y, ev = data
+    wl = workloads.config1()
+    pa, pb = host.Population(), host.Population()
+    for t in wl.trees[:90]:
+        pa.add(t)
+    for t in wl.trees[90:200]:
+        pb.add(t)
+    (ga, oa), (gb, ob) = pa.arrays(), pb.arrays()
+    want_a, ms_a0 = ev.eval_words(ga, oa, 0)
+    want_b, ms_b0 = ev.eval_words(gb, ob, 0)
+    assert ms_a0 >= 0 and ms_b0 >= 0
+    lib = ev._L
+    _say(f"pipelined eval: A {len(pa)} programs, B {len(pb)} programs")
+    out_a = np.zeros(len(pa), np.float32)
+    out_b = np.zeros(len(pb), np.float32)
+    assert lib.rc_set_programs(ev.ctx, L.ptr(ga, L.u32p), L.ptr(oa, L.u32p), len(pa)) == 0
+    assert lib.rc_eval_submit(ev.ctx, 0) == 0
+    assert lib.rc_set_programs(ev.ctx, L.ptr(gb, L.u32p), L.ptr(ob, L.u32p), len(pb)) == 0
+    ms_a = lib.rc_eval_collect(ev.ctx, L.ptr(out_a, L.f32p))
+    assert ms_a >= 0.0
+    assert np.array_equal(out_a.view(np.uint32), want_a.view(np.uint32))
+    assert lib.rc_eval_submit(ev.ctx, 0) == 0
+    ms_b = lib.rc_eval_collect(ev.ctx, L.ptr(out_b, L.f32p))
+    assert ms_b >= 0.0
+    assert np.array_equal(out_b.view(np.uint32), want_b.view(np.uint32))
+    # A synchronous score of the current batch between two pipelined steps.
+    assert lib.rc_set_programs(ev.ctx, L.ptr(ga, L.u32p), L.ptr(oa, L.u32p), len(pa)) == 0
+    assert lib.rc_eval_submit(ev.ctx, 0) == 0
+    sc, st, ms_s = ev.score_words(gb, ob, 0)
+    assert ms_s >= 0.0
+    ms_a2 = lib.rc_eval_collect(ev.ctx, L.ptr(out_a, L.f32p))
+    assert ms_a2 >= 0.0
+    assert np.array_equal(out_a.view(np.uint32), want_a.view(np.uint32))

@@ -128,3 +128,36 @@ def test_workloads_are_seeded_and_shaped():
     assert min(ks) >= 1 and max(ks) <= 20
     assert all(host.gpu_fit_plan(t) is not None for t in c3.trees)
     assert np.all(np.isfinite(c3.y))
+
+
+def _graph_key(code_words, nrowsets, nprog=64, ncands=64, nitems=128, nlist=256, kmax=8, tri=45):
+    import ctypes as C
+    L = lib.load()
+    buf = (C.c_ulonglong * 64)()
+    n = L.rch_fit_graph_key(code_words, ncands, nrowsets, nitems, nlist, nprog, kmax, tri, buf, 64)
+    assert n > 0
+    return tuple(buf[:n])
+
+
+def test_fit_graph_key_separates_regions_that_end_at_the_same_byte():
+    # Layouts whose region ends at the same byte (a shorter code block and one
+    # more row set) must not share a replay key: the graph bakes in the
+    # offsets of the candidate and row-set tables.
+    fields = 22                      # region offsets after the 7 shape words
+    dev_end_at = 7 + fields - 1
+    seen = {}
+    collisions = 0
+    for code_words in range(64, 64 * 40, 16):
+        for nrowsets in range(1, 18):
+            k = _graph_key(code_words, nrowsets)
+            offsets = k[7:7 + fields]
+            shape = (k[:7], k[dev_end_at])
+            for other in seen.get(shape, []):
+                if other[7:7 + fields] != offsets:
+                    collisions += 1
+                    assert other != k
+            seen.setdefault(shape, []).append(k)
+    # The shape words and region end alone would have replayed a stale graph.
+    assert collisions > 0
+    # Equal layouts give equal keys.
+    assert _graph_key(1000, 3) == _graph_key(1000, 3)

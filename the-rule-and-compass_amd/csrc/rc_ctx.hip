@@ -187,7 +187,8 @@ void *rc_create(const float *cols, int n, int v, const float *y)
     if (hipStreamCreateWithFlags(&c->s_eval, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->s_fit, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess ||
-        hipEventCreate(&c->tp) != hipSuccess ||
+        hipEventCreate(&c->tp) != hipSuccess || hipEventCreate(&c->tq0) != hipSuccess ||
+        hipEventCreate(&c->tq1) != hipSuccess || hipEventCreate(&c->tq_prep) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming) != hipSuccess) {
         (void)hipGetLastError();
         rc_rows_free(&c->sets[0]);
@@ -228,6 +229,9 @@ void rc_destroy(void *ctx)
     if (c->t0) (void)hipEventDestroy(c->t0);
     if (c->t1) (void)hipEventDestroy(c->t1);
     if (c->tp) (void)hipEventDestroy(c->tp);
+    if (c->tq0) (void)hipEventDestroy(c->tq0);
+    if (c->tq1) (void)hipEventDestroy(c->tq1);
+    if (c->tq_prep) (void)hipEventDestroy(c->tq_prep);
     if (c->ev_done) (void)hipEventDestroy(c->ev_done);
     delete c;
 }

@@ -1348,14 +1348,19 @@ int rc_eval_submit(void *ctx, int loss)
     if (!c || c->nprog <= 0 || c->pending) return -1;
     if (ensure_outputs(c, c->nprog) != 0) return -1;
     double t = rc_now_ms();
-    (void)hipEventRecord(c->t0, c->s_eval);
+    (void)hipEventRecord(c->tq0, c->s_eval);
     if (launch_rows(c, MODE_EVAL, loss, c->nprog, c->arena_code[c->arena_slot],
                     c->arena_meta[c->arena_slot], c->s_eval, nullptr) != 0) {
         c->stats.failed = 1;
         return -1;
     }
-    (void)hipEventRecord(c->t1, c->s_eval);
+    (void)hipEventRecord(c->tq1, c->s_eval);
     c->stats.launch_ms += rc_now_ms() - t;
+    /* The batch takes its upload's prep event with it: a later
+       rc_set_programs records into the other handle. */
+    c->pending_prep = c->prep_timed;
+    if (c->prep_timed) std::swap(c->tp, c->tq_prep);
+    c->prep_timed = 0;
     c->pending = 1;
     c->pending_nprog = c->nprog;
     return 0;
@@ -1367,7 +1372,7 @@ float rc_eval_collect(void *ctx, float *out)
     if (!c || !out || !c->pending) return -1.0f;
     RcStats *st = &c->stats;
     double t = rc_now_ms();
-    const hipError_t e = hipEventSynchronize(c->t1);
+    const hipError_t e = hipEventSynchronize(c->tq1);
     st->sync_ms += rc_now_ms() - t;
     c->pending = 0;
     if (e != hipSuccess || hipGetLastError() != hipSuccess) {
@@ -1383,8 +1388,10 @@ float rc_eval_collect(void *ctx, float *out)
     st->d2h_ms += rc_now_ms() - t;
     st->d2h_bytes += (unsigned long long)c->pending_nprog * 4ull;
     float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, c->prep_timed ? c->tp : c->t0, c->t1);
-    c->prep_timed = 0;
+    if (hipEventElapsedTime(&ms, c->pending_prep ? c->tq_prep : c->tq0, c->tq1) != hipSuccess) {
+        st->failed = 1;
+        return -1.0f;
+    }
     st->kernel_ms = ms;
     st->batch = c->pending_nprog;
     st->wall_ms = rc_now_ms() - c->batch_start_ms;

@@ -1141,6 +1141,64 @@ static size_t take(size_t &cur, size_t bytes)
     return o;
 }
 
+/* Byte offsets of the fit region: inputs [code | cands | rows | items |
+   lists | pre | c0], outputs [c | sse | status], then work.  The recorded
+   graph bakes every one of them into its kernel arguments, so the graph
+   key holds them all. */
+struct FtRegion {
+    size_t o_code, o_cand, o_rows, o_items, o_lists, o_pre, o_c0, in_end;
+    size_t o_outc, o_sse, o_st, io_end;
+    size_t o_state, o_run_a, o_run_t, o_cp32, o_cp64, o_pgram, o_ploss, o_pbad, o_pskip, dev_end;
+};
+static const int FT_REGION_FIELDS = (int)(sizeof(FtRegion) / sizeof(size_t));
+
+static FtRegion ft_region(size_t code_words, size_t ncands, size_t nrowsets, size_t nitems,
+                          size_t nlist_words, int nprog, int kmax, int tri_stride)
+{
+    FtRegion g;
+    size_t cur = 0;
+    const size_t np = (size_t)nprog, ni = std::max<size_t>(1, nitems);
+    g.o_code = take(cur, code_words * 4);
+    g.o_cand = take(cur, ncands * sizeof(FtCand));
+    g.o_rows = take(cur, nrowsets * sizeof(FtRows));
+    g.o_items = take(cur, ni * sizeof(FtItem));
+    g.o_lists = take(cur, nlist_words * 4);
+    g.o_pre = take(cur, np * 4);
+    g.o_c0 = take(cur, np * (size_t)kmax * 8);
+    g.in_end = take(cur, 0);
+    g.o_outc = take(cur, np * (size_t)kmax * 8);
+    g.o_sse = take(cur, np * 8);
+    g.o_st = take(cur, np * 4);
+    g.io_end = take(cur, 0);
+    g.o_state = take(cur, np * sizeof(FtState));
+    g.o_run_a = take(cur, np * 4);
+    g.o_run_t = take(cur, np * 4);
+    g.o_cp32 = take(cur, np * FT_KMAX * 4);
+    g.o_cp64 = take(cur, np * FT_KMAX * 8);
+    g.o_pgram = take(cur, ni * (size_t)tri_stride * 8);
+    g.o_ploss = take(cur, ni * 8);
+    g.o_pbad = take(cur, ni * 4);
+    g.o_pskip = take(cur, ni * 4);
+    g.dev_end = take(cur, 0);
+    return g;
+}
+
+/* The replay key of a recorded fit graph: the batch shape, the region base
+   and every region offset, then the class list offsets. */
+static std::vector<uint64_t> ft_graph_key(int nprog, int nitems, int iters, int any_irls, int kmax,
+                                          int tri_stride, const void *base, const FtRegion &g,
+                                          const int *lofs, int nl, const int *tofs, int nt)
+{
+    std::vector<uint64_t> key = {(uint64_t)nprog, (uint64_t)nitems, (uint64_t)iters,
+                                 (uint64_t)any_irls, (uint64_t)kmax, (uint64_t)tri_stride,
+                                 (uint64_t)(uintptr_t)base};
+    const size_t *fld = (const size_t *)&g;
+    for (int q = 0; q < FT_REGION_FIELDS; ++q) key.push_back((uint64_t)fld[q]);
+    for (int q = 0; q < nl; ++q) key.push_back((uint64_t)lofs[q]);
+    for (int q = 0; q < nt; ++q) key.push_back((uint64_t)tofs[q]);
+    return key;
+}
+
 static int fit_reserve(RcCtx *c, size_t dev_bytes, size_t host_bytes)
 {
     RcFitState *f = c->fit;
@@ -1411,29 +1469,14 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
 
     /* Region layout: inputs [code | cands | rows | items | lists | pre | c0]
        then outputs [c | sse | status] then work. */
-    size_t cur = 0;
-    const size_t o_code = take(cur, code.size() * 4);
-    const size_t o_cand = take(cur, cands.size() * sizeof(FtCand));
-    const size_t o_rows = take(cur, rowsd.size() * sizeof(FtRows));
-    const size_t o_items = take(cur, std::max<size_t>(1, items.size()) * sizeof(FtItem));
-    const size_t o_lists = take(cur, flat_lists.size() * 4);
-    const size_t o_pre = take(cur, (size_t)nprog * 4);
-    const size_t o_c0 = take(cur, (size_t)nprog * kmax * 8);
-    const size_t in_end = take(cur, 0);
-    const size_t o_outc = take(cur, (size_t)nprog * kmax * 8);
-    const size_t o_sse = take(cur, (size_t)nprog * 8);
-    const size_t o_st = take(cur, (size_t)nprog * 4);
-    const size_t io_end = take(cur, 0);
-    const size_t o_state = take(cur, (size_t)nprog * sizeof(FtState));
-    const size_t o_run_a = take(cur, (size_t)nprog * 4);
-    const size_t o_run_t = take(cur, (size_t)nprog * 4);
-    const size_t o_cp32 = take(cur, (size_t)nprog * FT_KMAX * 4);
-    const size_t o_cp64 = take(cur, (size_t)nprog * FT_KMAX * 8);
-    const size_t o_pgram = take(cur, (size_t)std::max(1, nitems) * tri_stride * 8);
-    const size_t o_ploss = take(cur, (size_t)std::max(1, nitems) * 8);
-    const size_t o_pbad = take(cur, (size_t)std::max(1, nitems) * 4);
-    const size_t o_pskip = take(cur, (size_t)std::max(1, nitems) * 4);
-    const size_t dev_end = take(cur, 0);
+    const FtRegion rg = ft_region(code.size(), cands.size(), rowsd.size(), items.size(),
+                                  flat_lists.size(), nprog, kmax, tri_stride);
+    const size_t o_code = rg.o_code, o_cand = rg.o_cand, o_rows = rg.o_rows, o_items = rg.o_items;
+    const size_t o_lists = rg.o_lists, o_pre = rg.o_pre, o_c0 = rg.o_c0, in_end = rg.in_end;
+    const size_t o_outc = rg.o_outc, o_sse = rg.o_sse, o_st = rg.o_st, io_end = rg.io_end;
+    const size_t o_state = rg.o_state, o_run_a = rg.o_run_a, o_run_t = rg.o_run_t;
+    const size_t o_cp32 = rg.o_cp32, o_cp64 = rg.o_cp64, o_pgram = rg.o_pgram, o_ploss = rg.o_ploss;
+    const size_t o_pbad = rg.o_pbad, o_pskip = rg.o_pskip, dev_end = rg.dev_end;
     double t = rc_now_ms();
     if (fit_reserve(c, dev_end, io_end) != 0) return -1.0f;
     c->stats.alloc_ms += rc_now_ms() - t;
@@ -1548,11 +1591,8 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     auto trial_pass = [&](bool exact) { mark(exact ? 2 : 1, [&]() { trial_pass0(exact); }); };
 
     /* The launch sequence: recorded as a hipGraph per batch shape. */
-    std::vector<uint64_t> key = {(uint64_t)nprog, (uint64_t)nitems, (uint64_t)iters,
-                                 (uint64_t)any_irls, (uint64_t)kmax, (uint64_t)tri_stride,
-                                 (uint64_t)dev_end, (uint64_t)(uintptr_t)D};
-    for (int q = 0; q <= NA; ++q) key.push_back((uint64_t)lofs[q]);
-    for (int q = 0; q <= NT; ++q) key.push_back((uint64_t)tofs[q]);
+    const std::vector<uint64_t> key = ft_graph_key(nprog, nitems, iters, any_irls, kmax, tri_stride,
+                                                   D, rg, lofs, NA + 1, tofs, NT + 1);
     const char *nograph = getenv("RCSR_FIT_NO_GRAPH");
     const bool use_graph = !(nograph && nograph[0] == '1') && !f->profile;
     auto record = [&]() {
@@ -1641,6 +1681,23 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     c->stats.wall_ms = rc_now_ms() - t_begin;
     c->fs_batches++;
     return ms;
+}
+
+extern "C" int rch_fit_graph_key(int code_words, int ncands, int nrowsets, int nitems,
+                                 int nlist_words, int nprog, int kmax, int tri_stride,
+                                 unsigned long long *key, int cap)
+{
+    if (code_words < 0 || ncands < 0 || nrowsets < 0 || nitems < 0 || nlist_words < 0 ||
+        nprog <= 0 || kmax < 0 || tri_stride < 0 || !key)
+        return -1;
+    const FtRegion g = ft_region((size_t)code_words, (size_t)ncands, (size_t)nrowsets,
+                                 (size_t)nitems, (size_t)nlist_words, nprog, kmax, tri_stride);
+    const int zeros[1] = {0};
+    const std::vector<uint64_t> k =
+        ft_graph_key(nprog, nitems, 30, 0, kmax, tri_stride, nullptr, g, zeros, 1, zeros, 1);
+    if ((int)k.size() > cap) return -2;
+    for (size_t q = 0; q < k.size(); ++q) key[q] = (unsigned long long)k[q];
+    return (int)k.size();
 }
 
 extern "C" {

@@ -137,6 +137,12 @@ struct RcCtx {
        when prep_timed is set. */
     hipEvent_t tp = nullptr;
     int prep_timed = 0;
+    /* Events owned by the pending submit/collect batch.  rc_eval_submit
+       swaps the upload's prep event into tq_prep, so an upload of the next
+       batch while this one runs records into a different event, and a
+       synchronous score call in between records its own t0/t1. */
+    hipEvent_t tq0 = nullptr, tq1 = nullptr, tq_prep = nullptr;
+    int pending_prep = 0;
     hipEvent_t ev_done = nullptr;
 
     /* Pending evaluator batch (submit/collect split). */

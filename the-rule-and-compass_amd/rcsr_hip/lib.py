@@ -106,6 +106,8 @@ SIGNATURES = {
                                       i32p, f64p, C.POINTER(RcMaxOut)]),
     "rch_eval_translate": (C.c_int, [u32p, u32p, C.c_int, C.c_int, u32p, C.c_int, u32p, C.c_int,
                                      u32p, C.c_int, i32p, i32p]),
+    "rch_fit_graph_key": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, C.c_int, u64p, C.c_int]),
 }
 
 _lib = None
