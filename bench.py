@@ -81,7 +81,10 @@ def main():
     ap.add_argument("--iters", type=int, default=30)
     ap.add_argument("--fp64-cands", type=int, default=-1,
                     help="candidates in the FP64-arm timing (-1: all)")
-    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--cpu-sample", type=int, default=20,
+                    help="CPU baseline sample: one candidate per constant count, at most this many")
+    ap.add_argument("--verify-per-k", type=int, default=3,
+                    help="C3 convergence / CPU-verification sample: candidates per constant count")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-eval", action="store_true",
@@ -158,10 +161,10 @@ def main():
     L.rc_fit_profile(ev.ctx, 1)
     fit()
     rep = lib.RcFitReport()
-    per = np.zeros(3 * len(plans), np.int32)
+    per = np.zeros(5 * len(plans), np.int32)
     L.rc_fit_report(ev.ctx, C.byref(rep), lib.ptr(per, lib.i32p), len(plans))
     L.rc_fit_profile(ev.ctx, 0)
-    per = per.reshape(-1, 3)
+    per = per.reshape(-1, 5)
     flops = float(np.sum(per[:, 0] * 2.0 * wl.n * (ks.astype(np.float64) ** 2 + ks)))
     asm_tflops = flops / (rep.asm_ms * 1e-3) / 1e12 if rep.asm_ms > 0 else 0.0
     # Exact-loss pass bytes: cols64 (3 x 8 B) + y64 (8 B) per row per candidate pass.
@@ -209,6 +212,18 @@ def main():
         "status_counts": {str(int(s)): int(np.sum(statuses == s)) for s in np.unique(statuses)},
     }
 
+    # Per-candidate LM accept/reject record of the last (profiled) mixed fit.
+    mixed_bits, mixed_nd = per[:, 3].copy(), per[:, 4].copy()
+
+    # Candidates stratified by constant count: up to `per_k` of each k, in
+    # batch order (the CPU legs run on such samples, so their k mix matches
+    # the batch's instead of its first rows').
+    def stratified(per_k):
+        pick = []
+        for kk in range(1, 21):
+            pick += [i for i in range(len(plans)) if ks[i] == kk][:per_k]
+        return sorted(pick)
+
     # The plain FP64 device arm on the same batch.
     if not args.no_fp64:
         nsub = len(plans) if args.fp64_cands < 0 else min(args.fp64_cands, len(plans))
@@ -216,6 +231,10 @@ def main():
         ts = time.perf_counter()
         f64outs, ms64 = ev.fit_batch(plans[:nsub], args.iters, 0, kmax=20)
         t64 = time.perf_counter() - ts
+        rep64 = lib.RcFitReport()
+        per64 = np.zeros(5 * nsub, np.int32)
+        L.rc_fit_report(ev.ctx, C.byref(rep64), lib.ptr(per64, lib.i32p), nsub)
+        per64 = per64.reshape(-1, 5)
         ev.set_fit_mode(host.FIT_MODE_MIXED)
         if ms64 < 0:
             raise RuntimeError("FP64 arm failed")
@@ -224,41 +243,75 @@ def main():
         okm = np.array([host.fit_status_usable(outs[i].status) and outs[i].status != 7 and
                         host.fit_status_usable(f64outs[i].status) and f64outs[i].status != 7
                         and np.isfinite(f64loss[i]) and f64loss[i] > 0 for i in range(nsub)])
-        rel = np.abs(mixed_loss[:nsub][okm] - f64loss[okm]) / f64loss[okm]
+        signed = (mixed_loss[:nsub][okm] - f64loss[okm]) / f64loss[okm]
+        rel = np.abs(signed)
+        st_m = statuses[:nsub]
+        st_f = np.asarray([o.status for o in f64outs])
         log(f"fp64 arm: {nsub} candidates in {t64:.2f}s -> {v64:.2f} cand/s; mixed vs fp64 "
             f"loss rel err median {np.median(rel):.2e}, p99 {np.quantile(rel, 0.99):.2e}, "
-            f"max {np.max(rel):.2e}")
-        result["fp64_device"] = {"value": v64, "unit": "candidates/s", "candidates": nsub,
-                                 "seconds": t64}
+            f"max {np.max(rel):.2e}; statuses agree {np.mean(st_m == st_f):.3f}")
+        # The first LM decision where the two arms part (their accept/reject
+        # records), for the candidates outside 1e-6.
+        idx_ok = np.nonzero(okm)[0]
+        diverge = []
+        for i in idx_ok[rel > 1e-6]:
+            a, b = int(mixed_bits[i]), int(per64[i, 3])
+            nd = min(int(mixed_nd[i]), int(per64[i, 4]), 31)
+            x = (a ^ b) & ((1 << nd) - 1)
+            diverge.append((x & -x).bit_length() - 1 if x else nd)
+        result["fp64_device"] = {
+            "value": v64, "unit": "candidates/s", "candidates": nsub, "seconds": t64,
+            "design": "same decomposition as the mixed arm (work items, hipGraph, control "
+                      "kernel); f64 rows, f64 value and adjoint sweeps, FP64-MFMA Gram, "
+                      "f64 trial losses"}
         result["speedup_vs_fp64"] = value / v64
         result["loss_rel_err_vs_fp64_device"] = {
             "median": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)),
             "max": float(np.max(rel)), "frac_le_1e-6": float(np.mean(rel <= 1e-6)),
-            "compared": int(okm.sum())}
+            "compared": int(okm.sum()),
+            "signed_quantiles": {q: float(np.quantile(signed, float(q)))
+                                 for q in ("0.01", "0.05", "0.5", "0.95", "0.99")},
+            "signed_min": float(signed.min()), "signed_max": float(signed.max()),
+            "frac_mixed_worse_gt_1e-6": float(np.mean(signed > 1e-6)),
+            "frac_mixed_better_gt_1e-6": float(np.mean(signed < -1e-6)),
+            "status_agreement": float(np.mean(st_m == st_f)),
+            "status_pairs": {f"{a}->{b}": int(np.sum((st_m == a) & (st_f == b)))
+                             for a in np.unique(st_m) for b in np.unique(st_f)
+                             if np.sum((st_m == a) & (st_f == b)) > 0},
+            "first_lm_decision_that_differs_outside_1e-6": {
+                str(d): int(diverge.count(d)) for d in sorted(set(diverge))}}
+        if not args.no_cpu and args.verify_per_k > 0:
+            result["c3_convergence_and_verification"] = verify_part(
+                args, wl, plans, ks, outs, f64outs, ev, stratified(args.verify_per_k))
 
-    # The CPU f64 oracle on a bounded sample of the same batch.
+    # The CPU f64 oracle on a bounded sample of the same batch: one candidate
+    # of each constant count 1..20.
     if not args.no_cpu and args.cpu_sample > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         threads = min(os.cpu_count() or 1, 16)
-        ns = min(args.cpu_sample, len(plans))
+        sample = stratified(1)[:args.cpu_sample]
+        ns = len(sample)
+        trees_s = [wl.trees[i] for i in sample]
         ts = time.perf_counter()
-        cl, _ = oracle.fit_batch(wl.trees[:ns], wl.x, wl.y, None, args.iters, threads=threads,
-                                 kmax=20)
+        cl, _ = oracle.fit_batch(trees_s, wl.x, wl.y, None, args.iters, threads=threads, kmax=20)
         tc = time.perf_counter() - ts
-        ok = np.isfinite(cl) & (cl > 0) & np.array([outs[i].status != 7 for i in range(ns)])
-        rel = np.abs(mixed_loss[:ns][ok] - cl[ok]) / cl[ok]
+        ok = np.isfinite(cl) & (cl > 0) & np.array([outs[i].status != 7 for i in sample])
+        rel = np.abs(mixed_loss[sample][ok] - cl[ok]) / cl[ok]
         log(f"cpu oracle: {ns} candidates on {threads} threads in {tc:.1f}s; mixed vs oracle "
             f"median {np.median(rel):.2e} max {np.max(rel):.2e}")
         # The reference's CPU fallback runs min(cores, 8) workers.
         w8 = min(os.cpu_count() or 1, 8)
         ts = time.perf_counter()
-        oracle.fit_batch(wl.trees[:ns], wl.x, wl.y, None, args.iters, threads=w8, kmax=20)
+        oracle.fit_batch(trees_s, wl.x, wl.y, None, args.iters, threads=w8, kmax=20)
         t8 = time.perf_counter() - ts
         log(f"cpu oracle at {w8} workers: {ns / t8:.2f} candidates/s")
+        kmix = ks[sample]
         result["cpu_baseline"] = {
             "value": ns / tc, "unit": "candidates/s", "cores": threads, "kind": "port",
-            "sample": f"first {ns} of the {len(plans)} candidates, full N={wl.n}, "
+            "sample": f"{ns} of the {len(plans)} candidates, one per constant count "
+                      f"k = {int(kmix.min())}..{int(kmix.max())} (sample mean k "
+                      f"{kmix.mean():.2f}, batch mean {ks.mean():.2f}), full N={wl.n}, "
                       f"{args.iters} LM iterations + 4 polish, one candidate per thread",
             "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
             "value_at_min_cores_8": ns / t8, "workers_at_min_cores_8": w8}
@@ -282,6 +335,83 @@ def main():
     if not args.no_arms:
         result["other_fit_arms"] = arms_part(host, workloads)
     print(json.dumps(result), flush=True)
+
+
+def verify_part(args, wl, plans, ks, outs, f64outs, ev, sample):
+    """On a k-stratified sample of the C3 batch: which candidates the f64 LM
+    has converged by the bench's iteration count (the FP64 arm at 120
+    iterations agrees to 1e-9), the mixed arm's loss error against the FP64
+    arm on the converged and unconverged ones, and the CPU verification of
+    both arms (refit_finalists_exact, fit/gpu.rs:666-687: the exact f64 LM
+    warm-started from each arm's constants) with the two verified Pareto
+    fronts over (loss, node count)."""
+    import math
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from rcsr_hip import host
+    sub = [plans[i] for i in sample]
+    ev.set_fit_mode(host.FIT_MODE_FP64)
+    long_outs, ms = ev.fit_batch(sub, 120, 0, kmax=20)
+    ev.set_fit_mode(host.FIT_MODE_MIXED)
+    if ms < 0:
+        raise RuntimeError("FP64 arm (120 iterations) failed")
+    rows = []
+    for j, i in enumerate(sample):
+        m, f, fl = outs[i], f64outs[i], long_outs[j]
+        usable = all(host.fit_status_usable(o.status) and o.status != 7 for o in (m, f, fl))
+        if not usable or not (f.sse > 0 and fl.sse > 0):
+            continue
+        conv = abs(f.sse - fl.sse) <= 1e-9 * fl.sse
+        rows.append((i, conv, (m.sse - f.sse) / f.sse))
+    conv = np.array([r[1] for r in rows])
+    err = np.array([r[2] for r in rows])
+    # CPU verification of both arms from their constants.
+    threads = min(os.cpu_count() or 1, 16)
+    def verified(res):
+        ts = [wl.trees[i].with_constants(res[i].c) if host.fit_status_usable(res[i].status)
+              else wl.trees[i] for i in sample]
+        l, _ = oracle.fit_batch(ts, wl.x, wl.y, None, args.iters, threads=threads, kmax=20)
+        return np.asarray(l)
+    t0 = time.perf_counter()
+    vm, vf = verified(outs), verified(f64outs)
+    tv = time.perf_counter() - t0
+    sizes = [plans[i].nodes for i in sample]
+    def front(scores):
+        idx = [j for j in range(len(sample)) if math.isfinite(scores[j])]
+        idx.sort(key=lambda j: (sizes[j], scores[j]))
+        out, best = [], math.inf
+        for j in idx:
+            if scores[j] < best * (1 - 1e-9):
+                out.append((int(sample[j]), int(sizes[j]), float(scores[j])))
+                best = scores[j]
+        return out
+    fm, ff = front(vm), front(vf)
+    same = len(fm) == len(ff) and all(a[0] == b[0] and a[1] == b[1] and
+                                      abs(a[2] - b[2]) <= 1e-6 * abs(b[2]) for a, b in zip(fm, ff))
+    fin = np.isfinite(vm) & np.isfinite(vf) & (vf > 0)
+    vrel = (vm[fin] - vf[fin]) / vf[fin]
+    log(f"C3 verification sample: {len(sample)} candidates, {int(conv.sum())} converged by "
+        f"{args.iters} iterations; mixed vs fp64 on converged max |rel| "
+        f"{np.max(np.abs(err[conv])) if conv.any() else 0:.2e}, unconverged median |rel| "
+        f"{np.median(np.abs(err[~conv])) if (~conv).any() else 0:.2e}; verified fronts "
+        f"identical: {same} ({len(fm)} points); CPU refit {tv:.1f}s")
+    return {
+        "sample": f"{len(sample)} candidates, up to {args.verify_per_k} per constant count",
+        "compared": len(rows),
+        "converged": int(conv.sum()),
+        "converged_max_abs_rel_err": float(np.max(np.abs(err[conv]))) if conv.any() else None,
+        "converged_frac_le_1e-6": float(np.mean(np.abs(err[conv]) <= 1e-6)) if conv.any() else None,
+        "unconverged_median_abs_rel_err": float(np.median(np.abs(err[~conv]))) if (~conv).any() else None,
+        "unconverged_signed_quantiles": {
+            q: float(np.quantile(err[~conv], float(q))) for q in ("0.05", "0.5", "0.95")}
+            if (~conv).any() else None,
+        "verified_rel_mixed_vs_fp64": {
+            "median": float(np.median(vrel)), "max": float(np.max(vrel)),
+            "min": float(np.min(vrel)), "frac_mixed_worse_gt_1e-6": float(np.mean(vrel > 1e-6))},
+        "verified_front_mixed": fm, "verified_front_fp64": ff,
+        "verified_fronts_identical": bool(same),
+        "cpu_refit_seconds": tv,
+    }
 
 
 def arms_part(host, workloads, ncand=256):

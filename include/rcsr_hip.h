@@ -273,9 +273,11 @@ typedef struct RcFitReport {
    previous setting or -1. */
 int rc_fit_profile(void *ctx, int enable);
 
-/* NEW: totals since rc_fit_profile, and (when per_cand is non-NULL) the
-   number of assembly / f32-trial / f64-exact passes each candidate of the
-   last fit consumed, three ints per candidate.  Returns 0 or -1. */
+/* NEW: totals since rc_fit_profile, and (when per_cand is non-NULL) five
+   ints per candidate of the last fit: the assembly, trial and f64-exact
+   passes it consumed, its LM accept/reject record (bit q set when the q-th
+   LM decision accepted the trial, for q < 31) and the number of LM
+   decisions.  Returns 0 or -1. */
 int rc_fit_report(void *ctx, RcFitReport *out, int *per_cand, int nprog);
 
 /* eval.cu:3836 rc_stats_begin / 3855 rc_last_stats. */

@@ -72,8 +72,8 @@ struct FtState {
     int k, metric, irls, rounds, inner, iters;
     int stage, it, round, assembled, first;
     int status, sstart, srank, improved, skipped;
-    int halvings, polish_it, pre_status, pad;
-    int n_asm, n_trial, n_exact, pad2; /* passes consumed (telemetry) */
+    int halvings, polish_it, pre_status, acc_bits; /* acc_bits: LM decision q accepted (bit q) */
+    int n_asm, n_trial, n_exact, n_decided; /* passes consumed, LM decisions (telemetry) */
 };
 
 /* Wave-uniform storage for the interpreter: an LLVM vector (registers,
@@ -85,6 +85,8 @@ template <typename T, int N> struct VStore;
 template <> struct VStore<float, 8> { fv8 v; __device__ float get(int i) const { return v[i]; } __device__ void set(int i, float x) { v[i] = x; } };
 template <> struct VStore<float, 32> { fv32 v; __device__ float get(int i) const { return v[i]; } __device__ void set(int i, float x) { v[i] = x; } };
 template <> struct VStore<double, 8> { dv8 v; __device__ double get(int i) const { return v[i]; } __device__ void set(int i, double x) { v[i] = x; } };
+typedef double dv32 __attribute__((ext_vector_type(32)));
+template <> struct VStore<double, 32> { dv32 v; __device__ double get(int i) const { return v[i]; } __device__ void set(int i, double x) { v[i] = x; } };
 template <typename T, int N> struct AStore {
     T a[N];
     __device__ T get(int i) const { return a[i]; }
@@ -102,15 +104,17 @@ template <> struct Stores<float, true> {
     typedef AStore<float, BIG_AS> Adj;
     typedef AStore<float, BIG_TAPE> Tape;
 };
+/* The f64 tapes serve the plain FP64 arm's assembly; the f64 value passes
+   never touch them. */
 template <> struct Stores<double, false> {
     typedef VStore<double, SMALL_S> Val;
     typedef VStore<double, SMALL_AS> Adj;
-    typedef AStore<double, 1> Tape;
+    typedef VStore<double, SMALL_TAPE> Tape;
 };
 template <> struct Stores<double, true> {
     typedef AStore<double, BIG_S> Val;
     typedef AStore<double, BIG_AS> Adj;
-    typedef AStore<double, 1> Tape;
+    typedef AStore<double, BIG_TAPE> Tape;
 };
 
 template <typename T> struct XVec;
@@ -187,11 +191,12 @@ __device__ __forceinline__ T ft_forward(const uint32_t *__restrict__ fc, int m,
 }
 
 /* Reverse sweep seeded with `seed` at the root.  Constant gradients go to
-   the LDS column block X (X[slot * XS + lane]); input gradients, when asked,
-   to gx.  Returns whether every constant gradient is finite. */
-template <typename T, bool BIG>
+   the LDS column block X (X[slot * XSTR + lane], in the block's element
+   type); input gradients, when asked, to gx.  Returns whether every constant
+   gradient is finite. */
+template <typename T, bool BIG, typename XT = float, int XSTR = XS>
 __device__ __forceinline__ bool ft_reverse(const uint32_t *__restrict__ rc, int nrev, T seed,
-                                           Tape2<T, BIG> &tape, float *X, int lane, T *gx)
+                                           Tape2<T, BIG> &tape, XT *X, int lane, T *gx)
 {
     typename Stores<T, BIG>::Adj ad;
     ad.set(0, seed);
@@ -209,7 +214,7 @@ __device__ __forceinline__ bool ft_reverse(const uint32_t *__restrict__ rc, int 
         T first = (T)0, second = (T)0;
         int npush = 0;
         if (kind == RK_CONST) {
-            if (X) X[arg * XS + lane] = (float)a;
+            if (X) X[arg * XSTR + lane] = (XT)a;
             fin = fin && isfinite(a);
         } else if (kind == RK_VAR) {
             if (gx) gx[arg] += a;
@@ -482,6 +487,114 @@ __global__ __launch_bounds__(64) void ft_asm_kernel(
     }
 }
 
+/* The plain FP64 arm's assembly on the general interpreter: ft_asm_kernel's
+   row contract with f64 rows, f64 constants, the f64 value and adjoint
+   sweeps, the augmented row [s*J | s*r | 0] in an f64 LDS block and the Gram
+   on FP64 MFMA (v_mfma_f64_16x16x4f64), accumulated in f64.  A row is usable
+   when its f64 prediction and residual are finite. */
+constexpr int XS64 = 68; /* f64 column stride: 64 rows + 4 */
+template <int KC, bool BIG, bool POLISH>
+__global__ __launch_bounds__(64) void ft_asm64_kernel(
+    const FtItem *__restrict__ items, const int *__restrict__ list, int nlist,
+    const FtCand *__restrict__ cands, const FtRows *__restrict__ rowsd,
+    const uint32_t *__restrict__ code, const int *__restrict__ run,
+    const double *__restrict__ cp64, double *__restrict__ pgram, int tri_stride,
+    double *__restrict__ ploss, int *__restrict__ pbad, int *__restrict__ pskip)
+{
+    typedef double double4v __attribute__((ext_vector_type(4)));
+    const int li = blockIdx.x;
+    if (li >= nlist) return;
+    const int item = list[li];
+    const FtItem it = items[item];
+    const int cand = it.cand;
+    if (!run[cand]) return;
+    const FtCand d = cands[cand];
+    const FtRows R = rowsd[d.rowset];
+    const int lane = threadIdx.x;
+    const int k = d.k;
+
+    __shared__ double X[KC * XS64];
+    for (int i = lane; i < KC * XS64; i += 64) X[i] = 0.0;
+    __syncthreads();
+
+    const double *cst = cp64 + (size_t)cand * FT_KMAX;
+    const uint32_t *fc = code + d.fwd_off;
+    const uint32_t *rc = code + d.rev_off;
+    double4v acc00 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
+    double lsum = 0.0;
+    int bad = 0, skip = 0;
+    const int row_end = it.row0 + it.rows;
+    for (int base = it.row0; base < row_end; base += 64) {
+        const int row = base + lane;
+        bool live = row < row_end;
+        const int rr = live ? row : it.row0;
+        const double wb = R.w64 ? R.w64[rr] : 1.0;
+        live = live && (wb > 0.0);
+        dv8 xd;
+        load_x(R.cols64, R.n, R.v, rr, xd);
+        Tape2<double, BIG> tape;
+        const double pred = ft_forward<double, BIG, true>(fc, d.m, cst, xd, R.cols64, R.n, rr, tape);
+        const double y64 = R.y64[rr];
+        const bool ok = live && isfinite(pred) && isfinite(pred - y64);
+        const RcRowFit rf = rc_row_fit(wb, pred, y64, d.metric);
+        const double seed = ok ? rf.s : 0.0;
+        const bool gfin = ft_reverse<double, BIG, double, XS64>(rc, d.nrev, seed, tape, X, lane,
+                                                                (double *)nullptr);
+        const bool use = ok && gfin;
+        if (POLISH) {
+            if (live && !ok) bad = 1;
+            if (ok && !gfin) ++skip;
+            if (ok) lsum += wb * rf.rho;
+        } else {
+            if (live && !use) bad = 1;
+            if (use) lsum += wb * rf.rho;
+        }
+        if (__any(ok && !gfin)) {
+            if (!use)
+                for (int j = 0; j < k; ++j) X[j * XS64 + lane] = 0.0;
+        }
+        X[k * XS64 + lane] = use ? rf.s * rf.r : 0.0;
+        X[(k + 1) * XS64 + lane] = 0.0;
+        __syncthreads();
+        const int col = lane & 15;
+        const int sub = lane >> 4;
+#pragma unroll 4
+        for (int q = 0; q < 16; ++q) {
+            const double a0 = X[col * XS64 + 4 * q + sub];
+            acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, acc00, 0, 0, 0);
+            if (KC == 32) {
+                const double a1 = X[(16 + col) * XS64 + 4 * q + sub];
+                acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a0, acc10, 0, 0, 0);
+                acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, acc11, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    /* FP64 MFMA output: lane l, register r holds D[l/16 + 4 r][l % 16]. */
+    const int K = k + 2;
+    double *pg = pgram + (size_t)item * (size_t)tri_stride;
+    const int j = lane & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = (lane >> 4) + 4 * r;
+        if (i >= j && i < K) pg[i * (i + 1) / 2 + j] = acc00[r];
+        if (KC == 32) {
+            const int i1 = 16 + i;
+            if (i1 < K) pg[i1 * (i1 + 1) / 2 + j] = acc10[r];
+            if (i >= j && i1 < K) pg[i1 * (i1 + 1) / 2 + 16 + j] = acc11[r];
+        }
+    }
+    const double ls = rc_wave_sum_f64(lsum);
+    int sk = skip;
+    for (int off = 32; off > 0; off >>= 1) sk += __shfl_xor(sk, off, 64);
+    const int anybad = __any(bad) ? 1 : 0;
+    if (lane == 0) {
+        ploss[item] = ls;
+        pbad[item] = anybad;
+        pskip[item] = sk;
+    }
+}
+
 /* ---- trial / exact loss: forward only ------------------------------------ */
 
 template <typename T, bool BIG>
@@ -584,10 +697,10 @@ __device__ __forceinline__ int ctrl_lane0(int op, int cand, FtState *s, const Ft
                                        const double *G, double sh_loss, int sh_bad, int sh_skip,
                                        int ran_trial, bool want_gram, double &solve_lambda,
                                        double *__restrict__ outc, double *__restrict__ outsse,
-                                       int *__restrict__ outstatus);
+                                       int *__restrict__ outstatus, int all64);
 __device__ __forceinline__ void ctrl_after_solve(int kind, int rc, int cand, FtState *s, int k,
                                               const double *delta, float *__restrict__ cp32,
-                                              double *__restrict__ cp64);
+                                              double *__restrict__ cp64, int all64);
 __device__ __forceinline__ void ctrl_flags(int op, int cand, const FtState *s,
                                         int *__restrict__ run_asm, int *__restrict__ run_trial);
 
@@ -597,7 +710,7 @@ __global__ __launch_bounds__(64) void ft_ctrl_kernel(
     float *__restrict__ cp32, double *__restrict__ cp64, const double *__restrict__ pgram,
     int tri_stride, const double *__restrict__ ploss, const int *__restrict__ pbad,
     const int *__restrict__ pskip, double *__restrict__ outc, double *__restrict__ outsse,
-    int *__restrict__ outstatus)
+    int *__restrict__ outstatus, int all64)
 {
     const int cand = blockIdx.x;
     if (cand >= ncand) return;
@@ -666,7 +779,7 @@ __global__ __launch_bounds__(64) void ft_ctrl_kernel(
     if (lane == 0) {
         double lam = 0.0;
         sh_solve = ctrl_lane0(op, cand, s, d, c0, kmax, cp32, cp64, G, sh_loss, sh_bad, sh_skip,
-                              ran_trial, want_gram, lam, outc, outsse, outstatus);
+                              ran_trial, want_gram, lam, outc, outsse, outstatus, all64);
         sh_lambda = lam;
     }
     __syncthreads();
@@ -674,7 +787,7 @@ __global__ __launch_bounds__(64) void ft_ctrl_kernel(
     if (solve != CT_SOLVE_NONE) {
         const int rc = ft_solve_wave(s->jtj, s->jtr, sh_lambda, k, sh_delta, L, sh_dsc, sh_rhs,
                                      lane);
-        if (lane == 0) ctrl_after_solve(solve, rc, cand, s, k, sh_delta, cp32, cp64);
+        if (lane == 0) ctrl_after_solve(solve, rc, cand, s, k, sh_delta, cp32, cp64, all64);
     }
     if (lane == 0) ctrl_flags(op, cand, s, run_asm, run_trial);
     __syncthreads();
@@ -708,12 +821,17 @@ __device__ __forceinline__ int ctrl_lane0(int op, int cand, FtState *s, const Ft
                                        const double *G, double sh_loss, int sh_bad, int sh_skip,
                                        int ran_trial, bool want_gram, double &solve_lambda,
                                        double *__restrict__ outc, double *__restrict__ outsse,
-                                       int *__restrict__ outstatus)
+                                       int *__restrict__ outstatus, int all64)
 {
     const int k = d.k;
     double *cc = s->c;
     int solve = CT_SOLVE_NONE;
-    auto set_pass32 = [&](const double *v) { ctrl_set_pass32(cp32, cand, k, v); };
+    /* The LM loop's passes read f32 constants, or f64 ones in the plain
+       FP64 arm (every pass on f64 rows). */
+    auto set_pass32 = [&](const double *v) {
+        if (all64) ctrl_set_pass64(cp64, cand, k, v);
+        else ctrl_set_pass32(cp32, cand, k, v);
+    };
     auto set_pass64 = [&](const double *v) { ctrl_set_pass64(cp64, cand, k, v); };
     auto take_gram = [&]() {
         for (int i = 0; i < k; ++i) {
@@ -770,6 +888,8 @@ __device__ __forceinline__ int ctrl_lane0(int op, int cand, FtState *s, const Ft
             s->n_asm = 0;
             s->n_trial = 0;
             s->n_exact = 0;
+            s->acc_bits = 0;
+            s->n_decided = 0;
             if (s->pre_status >= 0) {
                 s->stage = ST_DONE;
             } else if (s->iters > 0) {
@@ -810,6 +930,8 @@ __device__ __forceinline__ int ctrl_lane0(int op, int cand, FtState *s, const Ft
                 break;
             }
             bool brk = false;
+            if (tsse < s->best && s->n_decided < 31) s->acc_bits |= 1 << s->n_decided;
+            s->n_decided += 1;
             if (tsse < s->best) {
                 for (int j = 0; j < k; ++j) cc[j] = s->trial[j];
                 s->best = tsse;
@@ -906,7 +1028,7 @@ __device__ __forceinline__ int ctrl_lane0(int op, int cand, FtState *s, const Ft
    the current constants) or record the failed solve. */
 __device__ __forceinline__ void ctrl_after_solve(int kind, int rc, int cand, FtState *s, int k,
                                               const double *delta, float *__restrict__ cp32,
-                                              double *__restrict__ cp64)
+                                              double *__restrict__ cp64, int all64)
 {
     if (rc == 1) {
         if (kind == CT_SOLVE_LM) {
@@ -920,7 +1042,8 @@ __device__ __forceinline__ void ctrl_after_solve(int kind, int rc, int cand, FtS
     if (rc == 4) s->srank = 1;
     for (int j = 0; j < k; ++j) s->trial[j] = clamp_step(s->c[j] + delta[j]);
     if (kind == CT_SOLVE_LM) {
-        ctrl_set_pass32(cp32, cand, k, s->trial);
+        if (all64) ctrl_set_pass64(cp64, cand, k, s->trial);
+        else ctrl_set_pass32(cp32, cand, k, s->trial);
         s->halvings = s->irls ? 2 : 0;
         s->stage = ST_NEED_TRIAL;
     } else {
@@ -1129,11 +1252,6 @@ int rc_rows_ensure_f64(RcCtx *c, RcRows *r);
 extern "C" int rch_fitvm_compile(const unsigned int *rec, int words, int p, int nvars, int col_bytes,
                                  unsigned int flags, unsigned int *fwd, int cap_f,
                                  unsigned int *rev, int cap_r, int *info);
-int rc_fit64_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const int *pcount, int nprog,
-                 int kmax, const double *c0, const std::vector<int> &rowset,
-                 const std::vector<int> &metric, const std::vector<int> &pre, int iters,
-                 double *outc, double *outsse, int *outstatus);
-
 static size_t take(size_t &cur, size_t bytes)
 {
     size_t o = (cur + 255) & ~(size_t)255;
@@ -1187,11 +1305,12 @@ static FtRegion ft_region(size_t code_words, size_t ncands, size_t nrowsets, siz
    and every region offset, then the class list offsets. */
 static std::vector<uint64_t> ft_graph_key(int nprog, int nitems, int iters, int any_irls, int kmax,
                                           int tri_stride, const void *base, const FtRegion &g,
-                                          const int *lofs, int nl, const int *tofs, int nt)
+                                          const int *lofs, int nl, const int *tofs, int nt,
+                                          int mode)
 {
     std::vector<uint64_t> key = {(uint64_t)nprog, (uint64_t)nitems, (uint64_t)iters,
-                                 (uint64_t)any_irls, (uint64_t)kmax, (uint64_t)tri_stride,
-                                 (uint64_t)(uintptr_t)base};
+                                 (uint64_t)any_irls | ((uint64_t)mode << 8), (uint64_t)kmax,
+                                 (uint64_t)tri_stride, (uint64_t)(uintptr_t)base};
     const size_t *fld = (const size_t *)&g;
     for (int q = 0; q < FT_REGION_FIELDS; ++q) key.push_back((uint64_t)fld[q]);
     for (int q = 0; q < nl; ++q) key.push_back((uint64_t)lofs[q]);
@@ -1279,7 +1398,8 @@ struct FtVmCode {
    per lane, whose block has the 2-row column stride.  A stream that does not
    compile leaves its code nonzero: the candidate then runs on the general
    interpreter. */
-static void vm_compile(FtVmCode &o, const uint32_t *rec, int words, int p, int nv, int tune_rows)
+static void vm_compile(FtVmCode &o, const uint32_t *rec, int words, int p, int nv, int tune_rows,
+                       bool all64)
 {
     uint32_t vf[2 * 4 * FT_MAX_NODES], vr[2 * 8 * FT_MAX_NODES], vt[2 * 4 * FT_MAX_NODES];
     int info[6] = {0, 0, 0, 0, 0, 0}, tinfo[6] = {0, 0, 0, 0, 0, 0};
@@ -1289,10 +1409,12 @@ static void vm_compile(FtVmCode &o, const uint32_t *rec, int words, int p, int n
                                tinfo);
     o.vmrc = rch_fitvm_compile(rec, words, p, nv, VM_XS(4) * 4, VMC_INPUTS_IN_U, vf, nf, vr, nr,
                                info);
-    o.vm4 = o.vmrc == 0 && info[3] <= 8 && info[4] <= 1 && o.vorc == 0 && tune_rows != 2;
+    o.vm4 = o.vmrc == 0 && info[3] <= 8 && info[4] <= 1 && o.vorc == 0 && tune_rows != 2 && !all64;
+    /* The 2-row block: f32 columns, or the plain FP64 arm's f64 columns
+       (same register files, wider column stride). */
     if (o.vmrc == 0 && !o.vm4)
-        o.vmrc = rch_fitvm_compile(rec, words, p, nv, VM_XS(2) * 4, VMC_INPUTS_IN_U, vf, nf, vr,
-                                   nr, info);
+        o.vmrc = rch_fitvm_compile(rec, words, p, nv, all64 ? VM64_XS(2) * 8 : VM_XS(2) * 4,
+                                   VMC_INPUTS_IN_U, vf, nf, vr, nr, info);
     if (o.vorc == 0) o.vt.assign(vt, vt + 2 * tinfo[0]);
     if (o.vmrc == 0) {
         o.vf.assign(vf, vf + 2 * info[0]);
@@ -1330,8 +1452,12 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     /* Per-candidate translation and fit-VM compilation are independent:
        they run on host threads, and the streams are laid out in candidate
        order afterwards. */
-    const bool vm_pass = f->mode != 1;
-    std::vector<FtVmCode> vmc(vm_pass ? (size_t)nprog : 0);
+    /* Mode 1: the plain FP64 arm.  The same items, launch sequence and
+       control kernel run every row pass on f64 rows with f64 constants: the
+       assembly on the f64 kernels (FP64 MFMA Gram), the LM trial losses on
+       the exact-loss kernels. */
+    const bool all64 = f->mode == 1;
+    std::vector<FtVmCode> vmc((size_t)nprog);
     rc_parallel_for(nprog, 32, [&](int lo, int hi) {
         for (int i = lo; i < hi; ++i) {
             const int words = (int)(off[i + 1] - off[i]);
@@ -1341,22 +1467,12 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
             }
             const int nv = c->sets[rowset[i]].v;
             progs[i] = translate_fit(prog + off[i], words, pcount[i], nv, true, false);
-            if (vm_pass && progs[i].pre_status < 0)
-                vm_compile(vmc[i], prog + off[i], words, pcount[i], nv, c->tune_fit_rows);
+            if (progs[i].pre_status < 0)
+                vm_compile(vmc[i], prog + off[i], words, pcount[i], nv, c->tune_fit_rows, all64);
         }
     });
     for (int s = 0; s < c->nsets; ++s)
         if (rc_rows_ensure_f64(c, &c->sets[s]) != 0) return -1.0f;
-
-    if (f->mode == 1) {
-        std::vector<int> pre64((size_t)nprog);
-        for (int i = 0; i < nprog; ++i) pre64[i] = progs[i].pre_status;
-        if (rc_fit64_run(c, prog, off, pcount, nprog, kmax, c0, rowset, metric, pre64, iters, outc,
-                         outsse, outstatus) != 0)
-            return -1.0f;
-        c->fs_batches++;
-        return (float)c->stats.kernel_ms;
-    }
 
     /* Candidate descriptors, streams and work items. */
     std::vector<FtCand> cands((size_t)nprog);
@@ -1529,7 +1645,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     auto ctrl0 = [&](int op) {
         ft_ctrl_kernel<<<nprog, 64, 0, s>>>(op, nprog, d_state, d_cand, d_c0, kmax, d_run_a, d_run_t,
                                             d_cp32, d_cp64, d_pgram, tri_stride, d_ploss, d_pbad,
-                                            d_pskip, d_outc, d_sse, d_st);
+                                            d_pskip, d_outc, d_sse, d_st, all64 ? 1 : 0);
     };
     auto ctrl = [&](int op) { mark(3, [&]() { ctrl0(op); }); };
     auto asm_launch = [&](auto kern, int q) {
@@ -1543,8 +1659,34 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
                       d_rows, d_code, d_run_a, d_cp32, d_cp64, d_pgram, tri_stride, d_ploss,
                       d_pbad, d_pskip);
     };
+    auto asm64_launch = [&](auto kern, int q) {
+        const int nl = lofs[q + 1] - lofs[q];
+        if (nl <= 0) return;
+        kern<<<nl, 64, 0, s>>>(d_items, d_lists + lofs[q], nl, d_cand, d_rows, d_code, d_run_a,
+                               d_cp64, d_pgram, tri_stride, d_ploss, d_pbad, d_pskip);
+    };
+    auto vm_asm64 = [&](int kc, bool polish, int q) {
+        vm_launch_asm64(kc, polish, lofs[q + 1] - lofs[q], s, d_items, d_lists + lofs[q], d_cand,
+                        d_rows, d_code, d_run_a, d_cp64, d_pgram, tri_stride, d_ploss, d_pbad,
+                        d_pskip);
+    };
     auto asm_pass0 = [&](int polish) {
-        if (polish) {
+        if (all64) {
+            /* Classes 0 and 1 (4 f32 rows per lane) are empty here. */
+            vm_asm64(16, polish != 0, 2);
+            vm_asm64(32, polish != 0, 3);
+            if (polish) {
+                asm64_launch(ft_asm64_kernel<16, false, true>, 4);
+                asm64_launch(ft_asm64_kernel<32, false, true>, 5);
+                asm64_launch(ft_asm64_kernel<16, true, true>, 6);
+                asm64_launch(ft_asm64_kernel<32, true, true>, 7);
+            } else {
+                asm64_launch(ft_asm64_kernel<16, false, false>, 4);
+                asm64_launch(ft_asm64_kernel<32, false, false>, 5);
+                asm64_launch(ft_asm64_kernel<16, true, false>, 6);
+                asm64_launch(ft_asm64_kernel<32, true, false>, 7);
+            }
+        } else if (polish) {
             vm_asm(16, 4, true, 0);
             vm_asm(32, 4, true, 1);
             vm_asm(16, 2, true, 2);
@@ -1565,7 +1707,8 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         }
     };
     auto asm_pass = [&](int polish) { mark(0, [&]() { asm_pass0(polish); }); };
-    auto trial_pass0 = [&](bool exact) {
+    auto trial_pass0 = [&](bool exact_) {
+        const bool exact = exact_ || all64;
         for (int q = 0; q < NT; ++q) {
             const int nl = tofs[q + 1] - tofs[q];
             if (nl <= 0) continue;
@@ -1592,7 +1735,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
 
     /* The launch sequence: recorded as a hipGraph per batch shape. */
     const std::vector<uint64_t> key = ft_graph_key(nprog, nitems, iters, any_irls, kmax, tri_stride,
-                                                   D, rg, lofs, NA + 1, tofs, NT + 1);
+                                                   D, rg, lofs, NA + 1, tofs, NT + 1, f->mode);
     const char *nograph = getenv("RCSR_FIT_NO_GRAPH");
     const bool use_graph = !(nograph && nograph[0] == '1') && !f->profile;
     auto record = [&]() {
@@ -1694,7 +1837,7 @@ extern "C" int rch_fit_graph_key(int code_words, int ncands, int nrowsets, int n
                                  (size_t)nitems, (size_t)nlist_words, nprog, kmax, tri_stride);
     const int zeros[1] = {0};
     const std::vector<uint64_t> k =
-        ft_graph_key(nprog, nitems, 30, 0, kmax, tri_stride, nullptr, g, zeros, 1, zeros, 1);
+        ft_graph_key(nprog, nitems, 30, 0, kmax, tri_stride, nullptr, g, zeros, 1, zeros, 1, 0);
     if ((int)k.size() > cap) return -2;
     for (size_t q = 0; q < k.size(); ++q) key[q] = (unsigned long long)k[q];
     return (int)k.size();
@@ -1779,9 +1922,11 @@ int rc_fit_report(void *ctx, RcFitReport *out, int *per_cand, int nprog)
                       hipMemcpyDeviceToHost) != hipSuccess)
             return -1;
         for (int i = 0; i < nprog; ++i) {
-            per_cand[3 * i + 0] = h[i].n_asm;
-            per_cand[3 * i + 1] = h[i].n_trial;
-            per_cand[3 * i + 2] = h[i].n_exact;
+            per_cand[5 * i + 0] = h[i].n_asm;
+            per_cand[5 * i + 1] = h[i].n_trial;
+            per_cand[5 * i + 2] = h[i].n_exact;
+            per_cand[5 * i + 3] = h[i].acc_bits;
+            per_cand[5 * i + 4] = h[i].n_decided;
         }
     }
     return 0;

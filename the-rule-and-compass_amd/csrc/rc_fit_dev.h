@@ -195,117 +195,86 @@ __device__ __forceinline__ T ft_unary_partial(unsigned op, T x, T out)
 /* ln(1e-12): the smallest identified fraction of parameter space. */
 #define FT_LN_DET_FLOOR (-27.631021115928547)
 
-/* ---- f64 row evaluation of the post-order records ------------------------ */
+/* ---- f64 sweeps over the post-order records, one row per thread ---------- */
 
-/* Values of every node for one row in f64 (ft_forward_d, eval.cu:1324-1360):
-   v[i] is node i's value, v[m - 1] the prediction. */
-__device__ __forceinline__ void ft_forward64(const uint32_t *__restrict__ nd, int m,
-                                             const double *__restrict__ cols64, int n, int row,
-                                             const double *c, double *v)
+/* One record [op, left, right, slot] of a post-order fit program. */
+struct FtRec {
+    uint32_t op, left, right, slot;
+};
+__device__ __forceinline__ FtRec ft_rec(const uint32_t *__restrict__ recs, int i)
+{
+    const uint32_t *w = recs + 4 * i;
+    return {w[0], w[1], w[2], w[3]};
+}
+
+/* value_of (fit.rs:1211-1365) for every record of one row: val[i] is record
+   i's value, val[m - 1] the prediction. */
+__device__ __forceinline__ void ft_rec_values(const uint32_t *__restrict__ recs, int m,
+                                              const double *__restrict__ cols64, int n, int row,
+                                              const double *c, double *val)
 {
     for (int i = 0; i < m; ++i) {
-        const uint32_t op = nd[4 * i], k0 = nd[4 * i + 1], k1 = nd[4 * i + 2], slot = nd[4 * i + 3];
-        if (op == OP_CONST) v[i] = c[slot];
-        else if (op == OP_VAR) v[i] = cols64[(size_t)slot * (size_t)n + (size_t)row];
-        else if (ft_is_binary(op)) v[i] = ft_binary(op, v[k0], v[k1]);
-        else v[i] = ft_unary(op, v[k0]);
+        const FtRec r = ft_rec(recs, i);
+        double out;
+        switch (r.op) {
+            case OP_CONST: out = c[r.slot]; break;
+            case OP_VAR: out = cols64[(size_t)r.slot * (size_t)n + (size_t)row]; break;
+            default:
+                out = ft_is_binary(r.op) ? ft_binary(r.op, val[r.left], val[r.right])
+                                         : ft_unary(r.op, val[r.left]);
+        }
+        val[i] = out;
     }
 }
 
-/* Adjoint sweep of one row in f64 (ft_backward_d, eval.cu:1366-1463): g[j]
-   = d prediction / d constant j for the p constants; a node whose adjoint is
-   0 is skipped. */
-__device__ __forceinline__ void ft_backward64(const uint32_t *__restrict__ nd, int m, const double *v,
-                                              double *adj, double *g, int p)
+/* What a record with adjoint `a` hands to its children (fit.rs:1503-1569):
+   to_l for the left (or only) child, to_r for the right one.  Each
+   expression keeps the CPU fitter's operation order, so +, -, * and /
+   programs give the oracle's bits. */
+__device__ __forceinline__ void ft_rec_hand_down(uint32_t op, double a, double l, double r,
+                                                 double self, double &to_l, double &to_r)
 {
-    for (int i = 0; i < m; ++i) adj[i] = 0.0;
-    for (int j = 0; j < p; ++j) g[j] = 0.0;
+    to_r = 0.0;
+    switch (op) {
+        case OP_ADD: to_l = a; to_r = a; break;
+        case OP_SUB: to_l = a; to_r = -a; break;
+        case OP_MUL: to_l = a * r; to_r = a * l; break;
+        case OP_DIV: to_l = a / r; to_r = -a * l / (r * r); break;
+        case OP_POW:
+            to_l = 0.0;
+            if (l > 0.0) {
+                to_l = a * r * pow(l, r - 1.0);
+                to_r = a * pow(l, r) * log(l);
+            }
+            break;
+        default: to_l = a * ft_unary_partial<double>(op, l, self); break;
+    }
+}
+
+/* Reverse accumulation over the records of one row after ft_rec_values:
+   grad[j] = d prediction / d constant j for the p constants.  adj is scratch
+   for m values; a record whose adjoint is 0 hands nothing down. */
+__device__ __forceinline__ void ft_rec_gradient(const uint32_t *__restrict__ recs, int m,
+                                                const double *val, double *adj, double *grad,
+                                                int p)
+{
+    for (int j = 0; j < p; ++j) grad[j] = 0.0;
+    for (int i = 0; i + 1 < m; ++i) adj[i] = 0.0;
     adj[m - 1] = 1.0;
     for (int i = m - 1; i >= 0; --i) {
         const double a = adj[i];
         if (a == 0.0) continue;
-        const uint32_t op = nd[4 * i], k0 = nd[4 * i + 1], k1 = nd[4 * i + 2], slot = nd[4 * i + 3];
-        if (op == OP_CONST) { g[slot] += a; continue; }
-        if (op == OP_VAR) continue;
-        const double x = v[k0];
-        if (op == OP_ADD) { adj[k0] += a; adj[k1] += a; }
-        else if (op == OP_SUB) { adj[k0] += a; adj[k1] -= a; }
-        else if (op == OP_MUL) { adj[k0] += a * v[k1]; adj[k1] += a * x; }
-        else if (op == OP_DIV) { const double y = v[k1]; adj[k0] += a / y; adj[k1] += -a * x / (y * y); }
-        else if (op == OP_POW) {
-            const double y = v[k1];
-            if (x > 0.0) { adj[k0] += a * y * pow(x, y - 1.0); adj[k1] += a * pow(x, y) * log(x); }
-        } else {
-            adj[k0] += a * ft_unary_partial<double>(op, x, v[i]);
+        const FtRec r = ft_rec(recs, i);
+        if (r.op == OP_CONST) {
+            grad[r.slot] += a;
+        } else if (r.op != OP_VAR) {
+            double to_l, to_r;
+            const bool two = ft_is_binary(r.op);
+            ft_rec_hand_down(r.op, a, val[r.left], two ? val[r.right] : 0.0, val[i], to_l, to_r);
+            adj[r.left] += to_l;
+            if (two) adj[r.right] += to_r;
         }
     }
-}
-
-/* Damped, Jacobi-equilibrated Cholesky solve of (A~ + lambda diag + 1e-12)
-   delta~ = -D^-1/2 J^T r with the scale-free ln det gate on the undamped
-   system (ft_solve, eval.cu:1874-1949).  jtj is the packed lower triangle
-   (i*(i+1)/2 + j).  Returns 0 ok, 4 rank-deficient but solved, 1 no finite
-   step.  Serial: run by one lane.  Every multiply-subtract is an explicit
-   fma, so the wave-parallel form below produces the same bits. */
-__device__ int ft_solve(const double *jtj, const double *jtr, double lambda, int p, double *delta,
-                        double *L)
-{
-    double dscale[FT_KMAX];
-    for (int i = 0; i < p; ++i) {
-        const double aii = jtj[i * (i + 1) / 2 + i];
-        if (!(aii > 0.0) || !isfinite(aii)) return 1;
-        dscale[i] = sqrt(aii);
-    }
-    int rank_deficient = 0;
-    double sum_ln_L = 0.0;
-    int ok = 1;
-    for (int i = 0; i < p && ok; ++i) {
-        for (int j = 0; j <= i; ++j) {
-            double s = jtj[i * (i + 1) / 2 + j] / (dscale[i] * dscale[j]);
-            if (!isfinite(s)) return 1;
-            for (int q = 0; q < j; ++q) s = fma(-L[i * FT_KMAX + q], L[j * FT_KMAX + q], s);
-            if (i == j) {
-                if (!(s > 0.0) || !isfinite(s)) { ok = 0; break; }
-                L[i * FT_KMAX + i] = sqrt(s);
-                sum_ln_L = fma(0.5, log(s), sum_ln_L);
-            } else {
-                L[i * FT_KMAX + j] = s / L[j * FT_KMAX + j];
-            }
-        }
-    }
-    if (!ok) {
-        rank_deficient = 1;
-    } else {
-        const double ln_det = 2.0 * sum_ln_L;
-        if (!isfinite(ln_det) || ln_det < FT_LN_DET_FLOOR) rank_deficient = 1;
-    }
-    for (int i = 0; i < p; ++i) {
-        for (int j = 0; j <= i; ++j) {
-            double s = jtj[i * (i + 1) / 2 + j] / (dscale[i] * dscale[j]);
-            if (i == j) s = fma(s, 1.0 + lambda, 1e-12);
-            for (int q = 0; q < j; ++q) s = fma(-L[i * FT_KMAX + q], L[j * FT_KMAX + q], s);
-            if (i == j) {
-                if (!(s > 1e-18) || !isfinite(s)) return 1;
-                L[i * FT_KMAX + i] = sqrt(s);
-            } else {
-                L[i * FT_KMAX + j] = s / L[j * FT_KMAX + j];
-            }
-        }
-    }
-    double rhs[FT_KMAX];
-    for (int i = 0; i < p; ++i) rhs[i] = -jtr[i] / dscale[i];
-    for (int i = 0; i < p; ++i) {
-        double s = rhs[i];
-        for (int q = 0; q < i; ++q) s = fma(-L[i * FT_KMAX + q], rhs[q], s);
-        rhs[i] = s / L[i * FT_KMAX + i];
-    }
-    for (int i = p - 1; i >= 0; --i) {
-        double s = rhs[i];
-        for (int q = i + 1; q < p; ++q) s = fma(-L[q * FT_KMAX + i], delta[q], s);
-        delta[i] = s / L[i * FT_KMAX + i];
-    }
-    for (int i = 0; i < p; ++i) delta[i] /= dscale[i];
-    return rank_deficient ? 4 : 0;
 }
 
 /* (i, j), j <= i, of packed lower-triangle index t = i (i + 1) / 2 + j. */
@@ -321,11 +290,11 @@ __device__ __forceinline__ void ft_tri_index(int t, int &i, int &j)
 /* One Cholesky factorization of the scaled lower triangle in S (row-major,
    stride FT_KMAX), right-looking over the 64 lanes of a one-wave workgroup:
    at step j the diagonal is checked and square-rooted, column j is divided
-   by it and the trailing triangle receives its q = j update.  Each entry thus
-   gets ft_solve's updates in increasing q, then its divide or root: the same
-   bits as the serial row-by-row order.  Returns the first row whose diagonal
-   fails (p when none); lane 0 accumulates 0.5 ln s of the accepted diagonals
-   in row order. */
+   by it and the trailing triangle receives its q = j update.  Each entry
+   thus receives its fma updates in increasing q and then its divide or root,
+   so the factor does not depend on how the lanes share the work.  Returns
+   the first row whose diagonal fails (p when none); lane 0 accumulates
+   0.5 ln s of the accepted diagonals in row order. */
 template <bool UNDAMPED>
 __device__ int ft_chol_wave(double *S, int p, int lane, double &sum_ln)
 {
@@ -353,14 +322,20 @@ __device__ int ft_chol_wave(double *S, int p, int lane, double &sum_ln)
     return p;
 }
 
-/* ft_solve run by the 64 lanes of a one-wave workgroup, with the same result
-   bits and status: both factorizations are wave-parallel (ft_chol_wave), the
-   serial solve's early exits are resolved in its order (a non-finite scaled
-   entry in a row the serial pass reaches returns 1 before that row's
-   diagonal is judged), and the two triangular solves run on lane 0 in
-   ft_solve's order.  jtj, jtr in any memory; S (FT_KMAX^2), dsc and rhs
-   (FT_KMAX) are LDS scratch; delta (LDS, FT_KMAX) is complete in every lane
-   on return.  The status is the same in every lane. */
+/* The LM step solve (eval.cu:1874-1949) on the 64 lanes of a one-wave
+   workgroup: Jacobi equilibration by the diagonal of the packed lower
+   triangle jtj (i (i + 1) / 2 + j), the scale-free ln det gate on the
+   undamped equilibrated system (rank deficient below ln 1e-12), then the
+   damped system s (1 + lambda) + 1e-12 on the diagonal factorized and solved
+   for delta = -D^-1/2 (A~ + damping)^-1 D^-1/2 J^T r.  Returns 0, 4 (rank
+   deficient but solved) or 1 (no finite step: a non-positive or non-finite
+   diagonal, a non-finite scaled entry in a row the factorization reaches
+   before failing, or a damped pivot at or below 1e-18).  Both
+   factorizations are wave-parallel (ft_chol_wave); the triangular solves run
+   on lane 0 in row order with explicit fma.  jtj, jtr in any memory; S
+   (FT_KMAX^2), dsc and rhs (FT_KMAX) are LDS scratch; delta (LDS, FT_KMAX)
+   is complete in every lane on return.  The status is the same in every
+   lane. */
 __device__ int ft_solve_wave(const double *jtj, const double *jtr, double lambda, int p,
                              double *delta, double *S, double *dsc, double *rhs, int lane)
 {

@@ -433,51 +433,63 @@ __device__ __forceinline__ V vm_forward(const uint2 *__restrict__ fc, int nf,
 
    so the assembled system and every row flag are the same. */
 __device__ float vm_fmul_legacy(float a, float b) __asm("llvm.amdgcn.fmul.legacy");
+/* The adjoint product of one row: f32 rows use the legacy multiply; f64 rows
+   (the plain FP64 arm) select +0 for a zero adjoint, which is the same
+   contract (an Inf or NaN partial under a zero adjoint contributes 0). */
+__device__ __forceinline__ float vm_mul0(float a, float p) { return vm_fmul_legacy(a, p); }
+__device__ __forceinline__ double vm_mul0(double a, double p) { return a == 0.0 ? 0.0 : a * p; }
 template <typename V> __device__ __forceinline__ V vm_zmul(V a, V p, V &fin)
 {
+    typedef typename VmScalar<V>::T T;
     V r;
 #pragma unroll
     for (int q = 0; q < VmScalar<V>::R; ++q) {
-        r[q] = vm_fmul_legacy(a[q], p[q]);
-        fin[q] += a[q] * 0.0f;
+        r[q] = vm_mul0(a[q], p[q]);
+        fin[q] += a[q] * (T)0;
     }
     return r;
 }
-template <typename V> __device__ __forceinline__ V vm_zmul(V a, float c, V &fin)
+template <typename V> __device__ __forceinline__ V vm_zmul(V a, typename VmScalar<V>::T c, V &fin)
 {
+    typedef typename VmScalar<V>::T T;
     V r;
 #pragma unroll
     for (int q = 0; q < VmScalar<V>::R; ++q) {
-        r[q] = vm_fmul_legacy(a[q], c);
-        fin[q] += a[q] * 0.0f;
+        r[q] = vm_mul0(a[q], c);
+        fin[q] += a[q] * (T)0;
     }
     return r;
 }
 
-/* Gradient column write of the pack's rows (rows lane + 64 q of the block)
-   plus the finiteness accumulator (0 while every gradient is finite, NaN
-   after). */
+/* Gradient column write of the pack's rows (rows lane + 64 q of the block,
+   in the pack's element type) plus the finiteness accumulator (0 while
+   every gradient is finite, NaN after). */
 template <typename V> __device__ __forceinline__ void vm_gwrite(char *xl, uint32_t x, V g, V &fin)
 {
-    float *p = (float *)(xl + (x >> VMR_GOFS_SHIFT));
+    typedef typename VmScalar<V>::T T;
+    T *p = (T *)(xl + (x >> VMR_GOFS_SHIFT));
 #pragma unroll
     for (int q = 0; q < VmScalar<V>::R; ++q) {
         p[64 * q] = g[q];
-        fin[q] += g[q] * 0.0f;
+        fin[q] += g[q] * (T)0;
     }
 }
 
-/* Runs the reverse stream seeded with `seed` on a pack of f32 rows.  The
-   inputs are in U slots 0..v-1 (the stream was compiled with them there);
-   gradient columns go to the LDS block through xl (this lane's byte address
-   of row 0, column 0).  Returns the finiteness accumulator. */
+/* Runs the reverse stream seeded with `seed` on a pack of rows (f32, or f64
+   for the plain FP64 arm).  The inputs are in U slots 0..v-1 (the stream was
+   compiled with them there); gradient columns go to the LDS block through xl
+   (this lane's byte address of row 0, column 0; the stream's column offsets
+   were compiled for that block's stride and element size).  Returns the
+   finiteness accumulator. */
 template <typename V, typename FU, typename FB, typename FA>
 __device__ __forceinline__ V vm_reverse(const uint2 *__restrict__ rc, int nr, V seed,
-                                        const float *__restrict__ cst, const FU &U0, const FU &U1,
-                                        const FB &B0, const FB &B1, FA &A0, FA &A1, char *xl)
+                                        const typename VmScalar<V>::T *__restrict__ cst,
+                                        const FU &U0, const FU &U1, const FB &B0, const FB &B1,
+                                        FA &A0, FA &A1, char *xl)
 {
     typedef VmPack<V> P;
-    V a = seed, fin = vm_bcast<V>(0.0f);
+    typedef typename VmScalar<V>::T T;
+    V a = seed, fin = vm_bcast<V>((T)0);
     const uint32_t end = 8u * (uint32_t)nr;
     for (uint32_t o = 0; o < end; o += 8u) {
         const uint2 w = vm_word(rc, o);

@@ -16,6 +16,10 @@
    polish passes alike). */
 #define VM_XS(R) (64 * (R) + 2)
 
+/* LDS column stride (doubles) of the plain FP64 arm's assembly block of R
+   f64 rows per lane: the block's 64 R rows + 4. */
+#define VM64_XS(R) (64 * (R) + 4)
+
 /* Assembly over the items of one class list: kc 16 / 32 Gram columns,
    r = 2 or 4 rows per lane, polish = the f64-residual polish (with r = 4 the
    residual runs on the candidate's value-pass stream). */
@@ -23,6 +27,14 @@ void vm_launch_asm(int kc, int r, bool polish, int nl, hipStream_t s, const FtIt
                    const int *list, const FtCand *cands, const FtRows *rows, const uint32_t *code,
                    const int *run, const float *cp32, const double *cp64, double *pgram,
                    int tri_stride, double *ploss, int *pbad, int *pskip);
+
+/* The plain FP64 arm's assembly on the fit VM (2 f64 rows per lane, FP64
+   MFMA Gram): kc 16 / 32 Gram columns; polish selects the skip-and-count
+   rule for non-finite gradient rows. */
+void vm_launch_asm64(int kc, bool polish, int nl, hipStream_t s, const FtItem *items,
+                     const int *list, const FtCand *cands, const FtRows *rows,
+                     const uint32_t *code, const int *run, const double *cp64, double *pgram,
+                     int tri_stride, double *ploss, int *pbad, int *pskip);
 
 /* Trial (f32, 8 rows per lane) or exact (f64, 4 rows per lane) loss. */
 void vm_launch_trial(bool exact, int nl, hipStream_t s, const FtItem *items, const int *list,

@@ -309,6 +309,161 @@ __global__ __launch_bounds__(64) void vm_asm_kernel(
     }
 }
 
+/* Assembly of the plain FP64 arm on the fit VM: the same work items, row
+   contract and Gram layout as vm_asm_kernel, with every row quantity in f64.
+   Two f64 rows per lane (the vm_d2 pack: its register files have the slot
+   counts of the 2-row f32 block, so the streams are compiled for that shape
+   with this block's column stride), the value and adjoint sweeps on f64
+   rows with f64 constants, the augmented row [s*J | s*r | 0] in an f64 LDS
+   block and the Gram on FP64 MFMA (v_mfma_f64_16x16x4f64) accumulated in
+   f64 throughout.  The zero r_lo column keeps the control kernel's
+   J^T r = J^T r_hi + J^T r_lo exact.  A row is usable when its f64
+   prediction and residual are finite (no f32 magnitude rule). */
+template <int KC, bool POLISH>
+__global__ __launch_bounds__(64) void vm_asm64_kernel(
+    const FtItem *__restrict__ items, const int *__restrict__ list, int nlist,
+    const FtCand *__restrict__ cands, const FtRows *__restrict__ rowsd,
+    const uint32_t *__restrict__ code, const int *__restrict__ run,
+    const double *__restrict__ cp64, double *__restrict__ pgram, int tri_stride,
+    double *__restrict__ ploss, int *__restrict__ pbad, int *__restrict__ pskip)
+{
+    typedef vm_d2 V;
+    typedef dv16 FU;
+    typedef dv16 FB;
+    typedef dv8 FA;
+    typedef VmPack<V> P;
+    typedef double double4v __attribute__((ext_vector_type(4)));
+    constexpr int R = 2;
+    constexpr int XS = VM64_XS(R);
+    constexpr int KCA = (KC == 32) ? FT_KMAX + 2 : KC;
+    constexpr int BLOCK = 64 * R;
+    const int li = blockIdx.x;
+    if (li >= nlist) return;
+    const int item = list[li];
+    const FtItem it = items[item];
+    const int cand = it.cand;
+    if (!run[cand]) return;
+    const FtCand d = cands[cand];
+    const FtRows Rw = rowsd[d.rowset];
+    const int lane = threadIdx.x;
+    const int k = d.k;
+
+    __shared__ double X[KCA * XS];
+    for (int i = lane; i < KCA * XS; i += 64) X[i] = 0.0;
+    __syncthreads();
+
+    const double *cst = cp64 + (size_t)cand * FT_KMAX;
+    const uint2 *fc = (const uint2 *)(code + d.fwd_off);
+    const uint2 *rc = (const uint2 *)(code + d.rev_off);
+    char *xl = (char *)(X + lane);
+    double4v acc00 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
+    double lsum = 0.0;
+    int bad = 0, skip = 0;
+    FU U0 = (FU)(0.0), U1 = U0;
+    FB B0 = (FB)(0.0), B1 = B0;
+    FA A0 = (FA)(0.0), A1 = A0;
+    const int row_end = it.row0 + it.rows;
+    for (int base = it.row0; base < row_end; base += BLOCK) {
+        int q[R];
+        bool in[R];
+        double wb[R], yv[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = base + 64 * r + lane;
+            in[r] = row < row_end;
+            q[r] = in[r] ? row : it.row0;
+            wb[r] = Rw.w64 ? Rw.w64[q[r]] : 1.0;
+            yv[r] = Rw.y64[q[r]];
+        }
+#pragma unroll
+        for (int v = 0; v < VM_MAX_VARS; ++v)
+            if (v < Rw.v) {
+                const double *col = Rw.cols64 + (size_t)v * (size_t)Rw.n;
+                P::set(U0, U1, U1, U1, v, (V){col[q[0]], col[q[1]]});
+            }
+        const V pred = vm_forward<V, true>(fc, d.m, cst, U0, U1, U1, U1, U0, U1, U1, U1, B0, B1,
+                                           B1, B1);
+        bool live[R], ok[R];
+        double rs[R], rho[R];
+        V seed;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            live[r] = in[r] && wb[r] > 0.0;
+            ok[r] = live[r] && isfinite(pred[r]) && isfinite(pred[r] - yv[r]);
+            const RcRowFit f = rc_row_fit(wb[r], pred[r], yv[r], d.metric);
+            seed[r] = ok[r] ? f.s : 0.0;
+            rs[r] = f.s * f.r;
+            rho[r] = f.rho;
+        }
+        const V fin = vm_reverse<V, FU, FB, FA>(rc, d.nrev, seed, cst, U0, U1, B0, B1, A0, A1, xl);
+        bool fix = false;
+        bool use[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const bool gf = fin[r] == 0.0;
+            use[r] = ok[r] && gf;
+            if (POLISH) {
+                if (live[r] && !ok[r]) bad = 1;
+                skip += (ok[r] && !gf) ? 1 : 0;
+                if (ok[r]) lsum += wb[r] * rho[r];
+            } else {
+                if (live[r] && !use[r]) bad = 1;
+                if (use[r]) lsum += wb[r] * rho[r];
+            }
+            fix = fix || (ok[r] && !gf);
+        }
+        if (__any(fix)) {
+            for (int j = 0; j < k; ++j)
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (!use[r]) X[j * XS + 64 * r + lane] = 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) X[k * XS + 64 * r + lane] = use[r] ? rs[r] : 0.0;
+        __syncthreads();
+        const int col = lane & 15;
+        const int sub = lane >> 4;
+        const int col1 = (16 + col < KCA) ? 16 + col : KCA - 1;
+        const double *x0 = X + col * XS + sub;
+        const double *x1 = X + col1 * XS + sub;
+#pragma unroll 8
+        for (int g = 0; g < BLOCK / 4; ++g) {
+            const double a0 = x0[4 * g];
+            acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, acc00, 0, 0, 0);
+            if (KC == 32) {
+                const double a1 = x1[4 * g];
+                acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a0, acc10, 0, 0, 0);
+                acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, acc11, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    /* FP64 MFMA output: lane l, register r holds D[l/16 + 4 r][l % 16] of
+       each 16x16 block (not the FP32 form's 4 (l/16) + r). */
+    const int K = k + 2;
+    double *pg = pgram + (size_t)item * (size_t)tri_stride;
+    const int j = lane & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = (lane >> 4) + 4 * r;
+        if (i >= j && i < K) pg[i * (i + 1) / 2 + j] = acc00[r];
+        if (KC == 32) {
+            const int i1 = 16 + i;
+            if (i1 < K) pg[i1 * (i1 + 1) / 2 + j] = acc10[r];
+            if (i >= j && i1 < K) pg[i1 * (i1 + 1) / 2 + 16 + j] = acc11[r];
+        }
+    }
+    const double ls = rc_wave_sum_f64(lsum);
+    int sk = skip;
+    for (int off = 32; off > 0; off >>= 1) sk += __shfl_xor(sk, off, 64);
+    const int anybad = __any(bad) ? 1 : 0;
+    if (lane == 0) {
+        ploss[item] = ls;
+        pbad[item] = anybad;
+        pskip[item] = sk;
+    }
+}
+
 /* Trial / exact loss on the fit VM: the value-pass stream (inputs in U
    slots 0..v-1, no tape), 8 f32 rows or 4 f64 rows per lane, with an 8-slot
    U file of four 64-bit-element vectors. */
@@ -408,6 +563,25 @@ void vm_launch_asm(int kc, int r, bool polish, int nl, hipStream_t s, const FtIt
         else VM_ASM(32, false, 2);
     }
 #undef VM_ASM
+}
+
+void vm_launch_asm64(int kc, bool polish, int nl, hipStream_t s, const FtItem *items,
+                     const int *list, const FtCand *cands, const FtRows *rows,
+                     const uint32_t *code, const int *run, const double *cp64, double *pgram,
+                     int tri_stride, double *ploss, int *pbad, int *pskip)
+{
+    if (nl <= 0) return;
+#define VM_ASM64(KC, P)                                                                        \
+    vm_asm64_kernel<KC, P><<<nl, 64, 0, s>>>(items, list, nl, cands, rows, code, run, cp64, pgram, \
+                                             tri_stride, ploss, pbad, pskip)
+    if (polish) {
+        if (kc == 16) VM_ASM64(16, true);
+        else VM_ASM64(32, true);
+    } else {
+        if (kc == 16) VM_ASM64(16, false);
+        else VM_ASM64(32, false);
+    }
+#undef VM_ASM64
 }
 
 void vm_launch_trial(bool exact, int nl, hipStream_t s, const FtItem *items, const int *list,

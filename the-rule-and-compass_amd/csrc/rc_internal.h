@@ -38,6 +38,7 @@ struct RcRows {
 };
 
 struct RcFitState; /* rc_fit.hip */
+struct RcCorrState; /* rc_corr.hip */
 
 /* Buffers of the order / rank / maximum stages (rc_order.hip); device
    pointers grow with rc_grow, the host vectors stage asynchronous uploads. */
@@ -166,6 +167,7 @@ struct RcCtx {
     unsigned long long fs_batches = 0;
 
     RcFitState *fit = nullptr;
+    RcCorrState *corr = nullptr; /* the correlation arm's region (rc_corr.hip) */
 };
 
 /* Host helpers (rc_ctx.hip). */
@@ -186,6 +188,7 @@ void rc_parallel_for(int n, int min_chunk, const std::function<void(int, int)> &
    the stage buffers. */
 int rc_order_invalidate(RcCtx *c);
 /* rc_corr.hip: the correlation arm of rc_fit_batch (metrics 2 and 3). */
+void rc_corr_free(RcCtx *c);
 int rc_corr_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const int *pcount, int nprog,
                 int kmax, const double *c0, const int *gauge, int iters, int metric, double *outc,
                 double *outsse, int *outstatus);

@@ -67,7 +67,7 @@ __device__ __forceinline__ double seg_predict(const uint32_t *__restrict__ prog,
     const int set = seg / ntrials;
     const uint32_t *nd = prog + off[set];
     const int m = (int)((off[set + 1] - off[set]) / 4u);
-    ft_forward64(nd, m, cols64, n, row, c0 + (size_t)seg * (size_t)kmax, v);
+    ft_rec_values(nd, m, cols64, n, row, c0 + (size_t)seg * (size_t)kmax, v);
     return v[m - 1];
 }
 
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(MX_THREADS) void mx_scan_kernel(
     int mybad = 0; /* bit 0: a non-finite row; bit 1: the level is vetoed */
     for (int row = tid; row < n; row += MX_THREADS) {
         double v[FT_MAX_NODES];
-        ft_forward64(nd, m, cols64, n, row, c, v);
+        ft_rec_values(nd, m, cols64, n, row, c, v);
         const double pred = v[m - 1];
         const double r = pred - y64[row];
         if (!isfinite(pred) || fabs(pred) > 1e30 || !isfinite(r)) {
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(MX_THREADS) void mx_scan_kernel(
         if (want && !(mybad & 2)) {
             double adj[FT_MAX_NODES];
             double g[FT_KMAX];
-            ft_backward64(nd, m, v, adj, g, p);
+            ft_rec_gradient(nd, m, v, adj, g, p);
             const double b = g[coord];
             if (!isfinite(b)) {
                 mybad |= 1;
