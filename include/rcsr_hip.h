@@ -111,6 +111,21 @@ float rc_eval_collect(void *ctx, float *out);
 /* eval.cu:3429 rc_eval: submit + collect. */
 float rc_eval(void *ctx, float *out, int loss);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+   (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  The same losses as rc_set_programs(prog, off, nprog) followed
+   by rc_eval(out, loss), bit for bit, but the call allocates nothing: the
+   translated streams, partial sums and outputs live in the library's static
+   device array of 32 MiB per GPU (held until the process ends), and only
+   the derived columns the context already holds are used.  The context's
+   current batch is left as it was.  Calls that overlap on one GPU race on
+   that array.  Returns the device time in ms; -2 when the batch does not fit
+   the array (about nprog * ceil(n / 4096) * 12 bytes of partial sums plus
+   the streams), -1 on any other failure or while a submitted batch is
+   pending. */
+float rc_eval_static(void *ctx, const unsigned int *prog, const unsigned int *off, int nprog,
+                     float *out, int loss);
+
 /* eval.cu:3438 rc_predict: raw P x N f32 predictions, row-major. */
 int rc_predict(void *ctx, const unsigned int *prog, const unsigned int *off, int nprog,
                float *out);

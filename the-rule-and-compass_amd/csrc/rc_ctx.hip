@@ -195,6 +195,9 @@ void *rc_create(const float *cols, int n, int v, const float *y)
         delete c;
         return nullptr;
     }
+    /* The device address of rc_eval_static's array on this device (a failure
+       here only makes that entry point retry on its first call). */
+    (void)rc_static_scratch_init();
     return c;
 }
 

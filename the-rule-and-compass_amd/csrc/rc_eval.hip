@@ -43,6 +43,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -1016,22 +1017,55 @@ int tile_rows_for(int n, int tuned)
 template <int S, int R>
 void launch_class(int mode, dim3 grid, hipStream_t s, const RcCtx *c, const uint32_t *code,
                   const uint32_t *meta, const uint32_t *plist, int ncls, int tiles, int tile_rows,
-                  int kind, float *pred)
+                  int kind, float *pred, double *part, int *part_bad)
 {
     const RcRows *r = &c->sets[0];
     const RcScoreArgs sa = {r->w64, c->d_tsign, c->d_pbar, r->ybar, c->d_part2};
     if (mode == MODE_EVAL)
         rc_eval_kernel<S, R, MODE_EVAL><<<grid, 64, 0, s>>>(
             r->cols, c->d_derived, r->v, r->y, r->n, code, meta, plist, ncls, tiles, tile_rows, kind,
-            c->d_part, c->d_part_bad, pred, sa);
+            part, part_bad, pred, sa);
     else if (mode == MODE_SCORE)
         rc_eval_kernel<S, R, MODE_SCORE><<<grid, 64, 0, s>>>(
             r->cols, c->d_derived, r->v, r->y, r->n, code, meta, plist, ncls, tiles, tile_rows, kind,
-            c->d_part, c->d_part_bad, pred, sa);
+            part, part_bad, pred, sa);
     else
         rc_eval_kernel<S, R, MODE_PREDICT><<<grid, 64, 0, s>>>(
             r->cols, c->d_derived, r->v, r->y, r->n, code, meta, plist, ncls, tiles, tile_rows, kind,
-            c->d_part, c->d_part_bad, pred, sa);
+            part, part_bad, pred, sa);
+}
+
+/* The class launches of one row pass over a translated batch whose class
+   starts are cls[0..4], with explicit partial-sum buffers (nprog * tiles
+   items each). */
+int launch_classes(const RcCtx *c, int mode, int kind, int nprog, const uint32_t *code,
+                   const uint32_t *meta, const int *cls, int tiles, int tile_rows, hipStream_t s,
+                   float *pred, double *part, int *part_bad)
+{
+    for (int q = 0; q < 4; ++q) {
+        const int ncls = cls[q + 1] - cls[q];
+        if (ncls <= 0) continue;
+        const uint32_t *plist = meta + nprog + cls[q];
+        size_t grid;
+        if (tiles < 8) grid = (size_t)ncls * (size_t)tiles;
+        else grid = ((size_t)(tiles + 7) / 8) * (size_t)ncls * 8;
+        if (grid > 0x7fffffffu) return -1;
+        dim3 g((unsigned)grid);
+        if (q == 0)
+            launch_class<8, 4>(mode, g, s, c, code, meta, plist, ncls, tiles, tile_rows, kind, pred,
+                               part, part_bad);
+        else if (q == 1)
+            launch_class<16, 2>(mode, g, s, c, code, meta, plist, ncls, tiles, tile_rows, kind, pred,
+                                part, part_bad);
+        else if (q == 2)
+            launch_class<32, 1>(mode, g, s, c, code, meta, plist, ncls, tiles, tile_rows, kind, pred,
+                                part, part_bad);
+        else
+            launch_class<64, 1>(mode, g, s, c, code, meta, plist, ncls, tiles, tile_rows, kind, pred,
+                                part, part_bad);
+        if (hipGetLastError() != hipSuccess) return -1;
+    }
+    return 0;
 }
 
 int launch_rows(RcCtx *c, int mode, int kind, int nprog, const uint32_t *code,
@@ -1049,29 +1083,9 @@ int launch_rows(RcCtx *c, int mode, int kind, int nprog, const uint32_t *code,
         (rc_grow(c, (void **)&c->d_part2, &c->part2_cap, items, sizeof(double)) != 0 ||
          rc_grow(c, (void **)&c->d_pbar, &c->pbar_cap, (size_t)nprog, sizeof(double)) != 0))
         return -1;
-    for (int cls = 0; cls < 4; ++cls) {
-        const int ncls = c->cls_start[cls + 1] - c->cls_start[cls];
-        if (ncls <= 0) continue;
-        const uint32_t *plist = meta + nprog + c->cls_start[cls];
-        size_t grid;
-        if (tiles < 8) grid = (size_t)ncls * (size_t)tiles;
-        else grid = ((size_t)(tiles + 7) / 8) * (size_t)ncls * 8;
-        if (grid > 0x7fffffffu) return -1;
-        dim3 g((unsigned)grid);
-        if (cls == 0)
-            launch_class<8, 4>(mode, g, s, c, code, meta, plist, ncls, tiles, tile_rows, kind,
-                               pred);
-        else if (cls == 1)
-            launch_class<16, 2>(mode, g, s, c, code, meta, plist, ncls, tiles, tile_rows, kind,
-                               pred);
-        else if (cls == 2)
-            launch_class<32, 1>(mode, g, s, c, code, meta, plist, ncls, tiles, tile_rows, kind,
-                               pred);
-        else
-            launch_class<64, 1>(mode, g, s, c, code, meta, plist, ncls, tiles, tile_rows, kind,
-                               pred);
-        if (hipGetLastError() != hipSuccess) return -1;
-    }
+    if (launch_classes(c, mode, kind, nprog, code, meta, c->cls_start, tiles, tile_rows, s, pred,
+                       c->d_part, c->d_part_bad) != 0)
+        return -1;
     if (mode == MODE_PREDICT) return 0;
     rc_eval_finish<<<(nprog + 127) / 128, 128, 0, s>>>(c->d_part, c->d_part_bad, c->d_part2, nprog,
                                                        tiles, n, mode, kind, r->syy, c->d_out,
@@ -1295,9 +1309,118 @@ float score_current(RcCtx *c, int nprog, int kind, const int *pcount, double *ou
     return ms;
 }
 
+/* ---- rc_eval_static: the evaluation entry point that allocates nothing ----
+
+   Its device working memory (the translated streams, the per-(program,
+   tile) partial sums and the outputs) lives in one file-scope device array.
+   Each GPU holds its own copy at its own address, looked up once per device
+   with hipGetSymbolAddress while that device is current (rc_create's set-up,
+   or the first call on the device); host code uses only those addresses,
+   never the array's name.  Every call on a GPU shares that GPU's array: the
+   last call's contents stay, and calls that overlap on one GPU race.  No
+   part of the array needs clearing: every partial sum and output is written
+   before it is read. */
+constexpr size_t RC_STATIC_BYTES = (size_t)32 << 20;
+constexpr int RC_STATIC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[RC_STATIC_BYTES];
+void *g_static_ptr[RC_STATIC_MAX_DEVICES];
+std::mutex g_static_mu;
+
+size_t static_take(size_t &cur, size_t bytes)
+{
+    const size_t o = (cur + 255) & ~(size_t)255;
+    cur = o + bytes;
+    return o;
+}
+
 } /* namespace */
 
+/* Look up this device's copy of the static array (the current device). */
+int rc_static_scratch_init()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= RC_STATIC_MAX_DEVICES) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    if (g_static_ptr[dev]) return 0;
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    g_static_ptr[dev] = ptr;
+    return 0;
+}
+
 extern "C" {
+
+float rc_eval_static(void *ctx, const unsigned int *prog, const unsigned int *off, int nprog,
+                     float *out, int loss)
+{
+    RcCtx *c = (RcCtx *)ctx;
+    if (!c || !prog || !off || !out || nprog <= 0 || c->pending) return -1.0f;
+    RcRows *r = &c->sets[0];
+    const int n = r->n;
+    if (n <= 0) return -1.0f;
+    if (rc_static_scratch_init() != 0) return -1.0f;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1.0f;
+    unsigned char *base = (unsigned char *)g_static_ptr[dev];
+    /* The batch's streams, with the derived columns the context already
+       holds and no new ones (a new column would need device memory). */
+    std::vector<uint32_t> code, meta;
+    std::vector<uint32_t> derived = c->derived;
+    int cls[5];
+    (void)rc_translate_programs(prog, off, nprog, r->v, code, meta, cls, &derived, c->derived_cols);
+    const int tile_rows = tile_rows_for(n, c->tune_eval_tile);
+    const int tiles = (n + tile_rows - 1) / tile_rows;
+    const size_t items = (size_t)nprog * (size_t)tiles;
+    size_t cur = 0;
+    const size_t o_code = static_take(cur, code.size() * 4);
+    const size_t o_meta = static_take(cur, meta.size() * 4);
+    const size_t o_part = static_take(cur, items * sizeof(double));
+    const size_t o_bad = static_take(cur, items * sizeof(int));
+    const size_t o_out = static_take(cur, (size_t)nprog * sizeof(float));
+    if (cur > RC_STATIC_BYTES) return -2.0f; /* the batch does not fit the static array */
+    uint32_t *d_code = (uint32_t *)(base + o_code);
+    uint32_t *d_meta = (uint32_t *)(base + o_meta);
+    double *d_part = (double *)(base + o_part);
+    int *d_bad = (int *)(base + o_bad);
+    float *d_out = (float *)(base + o_out);
+    hipStream_t s = c->s_eval;
+    (void)hipEventRecord(c->t0, s);
+    if (hipMemcpyAsync(d_code, code.data(), code.size() * 4, hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(d_meta, meta.data(), meta.size() * 4, hipMemcpyHostToDevice, s) != hipSuccess)
+        return -1.0f;
+    const size_t nhoist = (meta.size() - 2 * (size_t)nprog) / 2;
+    if (nhoist > 0x7fffffffu) return -1.0f;
+    if (nhoist > 0) {
+        rc_hoist_kernel<<<(unsigned)nhoist, 64, 0, s>>>(d_code, d_meta + 2 * (size_t)nprog,
+                                                        (int)nhoist);
+        if (hipGetLastError() != hipSuccess) return -1.0f;
+    }
+    if (launch_classes(c, MODE_EVAL, loss, nprog, d_code, d_meta, cls, tiles, tile_rows, s, nullptr,
+                       d_part, d_bad) != 0)
+        return -1.0f;
+    rc_eval_finish<<<(nprog + 127) / 128, 128, 0, s>>>(d_part, d_bad, nullptr, nprog, tiles, n,
+                                                       MODE_EVAL, loss, r->syy, d_out, nullptr,
+                                                       nullptr, nullptr);
+    (void)hipEventRecord(c->t1, s);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(out, d_out, (size_t)nprog * sizeof(float), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipGetLastError();
+        return -1.0f;
+    }
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, c->t0, c->t1);
+    return ms;
+}
+
 
 int rch_eval_translate(const unsigned int *prog, const unsigned int *off, int nprog, int nvars,
                        unsigned int *code, int cap_code, unsigned int *meta, int cap_meta,

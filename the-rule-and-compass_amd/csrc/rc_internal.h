@@ -189,6 +189,9 @@ void rc_parallel_for(int n, int min_chunk, const std::function<void(int, int)> &
 int rc_order_invalidate(RcCtx *c);
 /* rc_corr.hip: the correlation arm of rc_fit_batch (metrics 2 and 3). */
 void rc_corr_free(RcCtx *c);
+/* rc_eval.hip: look up the current device's copy of rc_eval_static's device
+   array (once per device).  Returns 0 or -1. */
+int rc_static_scratch_init();
 int rc_corr_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const int *pcount, int nprog,
                 int kmax, const double *c0, const int *gauge, int iters, int metric, double *outc,
                 double *outsse, int *outstatus);

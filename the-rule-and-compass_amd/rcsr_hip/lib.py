@@ -106,6 +106,7 @@ SIGNATURES = {
                                       i32p, f64p, C.POINTER(RcMaxOut)]),
     "rch_eval_translate": (C.c_int, [u32p, u32p, C.c_int, C.c_int, u32p, C.c_int, u32p, C.c_int,
                                      u32p, C.c_int, i32p, i32p]),
+    "rc_eval_static": (C.c_float, [vp, u32p, u32p, C.c_int, f32p, C.c_int]),
     "rch_fit_graph_key": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_int, u64p, C.c_int]),
 }
