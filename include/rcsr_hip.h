@@ -146,6 +146,25 @@ float rc_pool_score(void *ctx, int which, int nprog, int width, int kind, const 
                     double *out, int *status);
 int rc_pool_hash(void *ctx, int which, int nprog, int width, unsigned long long *out);
 
+/* One generation of the flat postfix variation (postfix.rs:500-893, host
+   code in the reference; Philox4x32-10 of genome.rs:214-300) computed on the
+   device from pool arena `from` into the other arena: per island the best
+   parent survives in slot 0, islands with restart[i] != 0 are refilled with
+   random programs, every other slot is a child keyed by (seed, island,
+   generation, slot, attempt).  ops holds nops (code, arity) pairs; err holds
+   the parents' errors (NaN ranks last).  The optional outputs receive each
+   slot's length in words, node count, FNV-1a hash and inherited error (NaN
+   for new programs).  Returns 0, or -1 on an invalid argument or a device
+   error; the pool width must be at most 128 words. */
+int rc_pool_vary(void *ctx, int from, int islands, int pop, unsigned int seed,
+                 unsigned int generation, int max_complexity, int nvars, int init_depth,
+                 int dedup, const unsigned int *ops, int nops, const double *err,
+                 const unsigned char *restart, unsigned short *out_len, unsigned short *out_cost,
+                 unsigned long long *out_hash, double *out_err);
+/* Copies the first nprog slots of arena `which` to out (nprog * width
+   words).  Returns 0 or -1. */
+int rc_pool_download(void *ctx, int which, unsigned int *out, int nprog);
+
 /* ---- fit --------------------------------------------------------------- */
 
 /* eval.cu:4001 rc_fit_batch: batched LM / IRLS constant fitting.  prog holds

@@ -76,6 +76,15 @@ def lib():
         L.or_rank_sums.restype = None
         L.or_rank_sums.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.c_int,
                                    C.POINTER(C.c_ulonglong)]
+        L.or_philox.restype = None
+        L.or_philox.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                C.POINTER(C.c_uint32)]
+        L.or_vary.restype = C.c_longlong
+        L.or_vary.argtypes = [C.POINTER(C.c_uint32), C.c_int, C.c_int, C.c_int, _dp,
+                              C.POINTER(C.c_ubyte), C.c_uint32, C.c_uint32, C.c_int, C.c_int,
+                              C.c_int, C.c_int, C.POINTER(C.c_uint32), C.c_int,
+                              C.POINTER(C.c_uint32), C.POINTER(C.c_uint16),
+                              C.POINTER(C.c_uint16), C.POINTER(C.c_uint64), _dp]
         L.or_max_scan.restype = None
         L.or_max_scan.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp, _ip, _dp, _dp, _ip]
         _lib = L
@@ -328,3 +337,47 @@ def corr_score(tree, X, y, w=None, metric=2) -> float:
     y = np.ascontiguousarray(y, np.float64)
     return lib().or_corr_score(_i(ops), _d(vals), len(ops), _d(X), X.shape[0], X.shape[1], _d(y),
                                _d(_w(w)), int(metric))
+
+
+def philox(key, ctr):
+    """Philox4x32-10 of one (key, counter) block (genome.rs:214-300)."""
+    k = np.ascontiguousarray(key, np.uint32)
+    c = np.ascontiguousarray(ctr, np.uint32)
+    out = np.zeros(4, np.uint32)
+    u32 = C.POINTER(C.c_uint32)
+    lib().or_philox(k.ctypes.data_as(u32), c.ctypes.data_as(u32), out.ctypes.data_as(u32))
+    return [int(v) for v in out]
+
+
+# The reference's default operator set (expr.rs:392-398) as (opcode, arity)
+# pairs in table order: neg exp ln abs sqrt sin cos tan sinh cosh tanh
+# logistic gauss square cube add sub mul div pow if.
+DEFAULT_OPS = [(17, 1), (9, 1), (10, 1), (12, 1), (11, 1), (6, 1), (7, 1), (8, 1), (21, 1),
+               (22, 1), (13, 1), (14, 1), (18, 1), (15, 1), (16, 1), (2, 2), (3, 2), (4, 2),
+               (5, 2), (20, 2), (32, 3)]
+
+
+def vary(parents, islands, pop, err, seed, gen, max_complexity=30, nvars=2, init_depth=3,
+         dedup=True, restart=None, ops=None):
+    """One generation of the flat variation engine (postfix.rs:500-893)."""
+    parents = np.ascontiguousarray(parents, np.uint32)
+    n, width = parents.shape
+    assert n == islands * pop
+    err = np.ascontiguousarray(err, np.float64)
+    rs = None if restart is None else np.ascontiguousarray(restart, np.uint8)
+    table = np.ascontiguousarray(np.asarray(ops or DEFAULT_OPS, np.uint32).reshape(-1))
+    words = np.zeros((n, width), np.uint32)
+    ln = np.zeros(n, np.uint16)
+    cost = np.zeros(n, np.uint16)
+    h = np.zeros(n, np.uint64)
+    e = np.zeros(n, np.float64)
+    u32 = C.POINTER(C.c_uint32)
+    att = lib().or_vary(parents.ctypes.data_as(u32), islands, pop, width, _d(err),
+                        None if rs is None else rs.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                        seed, gen, max_complexity, nvars, init_depth, int(dedup),
+                        table.ctypes.data_as(u32), len(table) // 2, words.ctypes.data_as(u32),
+                        ln.ctypes.data_as(C.POINTER(C.c_uint16)),
+                        cost.ctypes.data_as(C.POINTER(C.c_uint16)),
+                        h.ctypes.data_as(C.POINTER(C.c_uint64)), _d(e))
+    assert att >= 0
+    return {"words": words, "len": ln, "cost": cost, "hash": h, "err": e, "attempts": int(att)}

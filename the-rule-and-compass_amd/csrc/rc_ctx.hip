@@ -227,6 +227,7 @@ void rc_destroy(void *ctx)
     if (c->h_out) (void)hipHostFree(c->h_out);
     rc_fit_state_free(c);
     rc_corr_free(c);
+    rc_vary_free(c);
     rc_order_free(c);
     if (c->s_eval) (void)hipStreamDestroy(c->s_eval);
     if (c->s_fit) (void)hipStreamDestroy(c->s_fit);

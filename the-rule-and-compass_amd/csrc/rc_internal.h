@@ -39,6 +39,7 @@ struct RcRows {
 
 struct RcFitState; /* rc_fit.hip */
 struct RcCorrState; /* rc_corr.hip */
+struct RcVaryState; /* rc_vary.hip */
 
 /* Buffers of the order / rank / maximum stages (rc_order.hip); device
    pointers grow with rc_grow, the host vectors stage asynchronous uploads. */
@@ -168,6 +169,7 @@ struct RcCtx {
 
     RcFitState *fit = nullptr;
     RcCorrState *corr = nullptr; /* the correlation arm's region (rc_corr.hip) */
+    RcVaryState *vary = nullptr; /* the device variation's region (rc_vary.hip) */
 };
 
 /* Host helpers (rc_ctx.hip). */
@@ -189,6 +191,7 @@ void rc_parallel_for(int n, int min_chunk, const std::function<void(int, int)> &
 int rc_order_invalidate(RcCtx *c);
 /* rc_corr.hip: the correlation arm of rc_fit_batch (metrics 2 and 3). */
 void rc_corr_free(RcCtx *c);
+void rc_vary_free(RcCtx *c);
 /* rc_eval.hip: look up the current device's copy of rc_eval_static's device
    array (once per device).  Returns 0 or -1. */
 int rc_static_scratch_init();

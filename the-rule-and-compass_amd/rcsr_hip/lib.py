@@ -18,6 +18,8 @@ i32p = C.POINTER(C.c_int)
 f32p = C.POINTER(C.c_float)
 f64p = C.POINTER(C.c_double)
 u64p = C.POINTER(C.c_ulonglong)
+u8p = C.POINTER(C.c_ubyte)
+u16p = C.POINTER(C.c_uint16)
 vp = C.c_void_p
 
 
@@ -69,6 +71,10 @@ SIGNATURES = {
     "rc_pool_eval": (C.c_float, [vp, C.c_int, C.c_int, C.c_int, f32p, C.c_int]),
     "rc_pool_score": (C.c_float, [vp, C.c_int, C.c_int, C.c_int, C.c_int, i32p, f64p, i32p]),
     "rc_pool_hash": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, u64p]),
+    "rc_pool_vary": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_uint, C.c_uint, C.c_int, C.c_int,
+                               C.c_int, C.c_int, u32p, C.c_int, f64p, u8p, u16p, u16p, u64p,
+                               f64p]),
+    "rc_pool_download": (C.c_int, [vp, C.c_int, u32p, C.c_int]),
     "rc_fit_batch": (C.c_float, [vp, u32p, u32p, i32p, C.c_int, C.c_int, f64p, i32p, C.c_int,
                                  C.c_int, f64p, f64p, i32p]),
     "rc_fit_kernel_info": (C.c_int, [C.c_int, i32p, i32p, i32p]),
