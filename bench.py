@@ -101,6 +101,13 @@ def main():
                     help="skip the exact order / rank / maximum stages part")
     ap.add_argument("--order-only", action="store_true",
                     help="only the exact order / rank / maximum stages (one JSON object)")
+    ap.add_argument("--no-probe", action="store_true",
+                    help="skip the dispatch device-rate probe")
+    ap.add_argument("--no-implicit", action="store_true",
+                    help="skip the implicit-relation fit part")
+    ap.add_argument("--implicit-cands", type=int, default=2048)
+    ap.add_argument("--implicit-cpu", type=int, default=7,
+                    help="implicit CPU-oracle sample: one candidate per constant count 2..8")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device copy of 256 MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -334,6 +341,15 @@ def main():
     # The logistic and correlation fit arms on a sample of the C3 batch.
     if not args.no_arms:
         result["other_fit_arms"] = arms_part(host, workloads)
+    # The dispatch cost model's device rates, probed on this device at the
+    # reference's calibration shape, and the fit route they give this batch.
+    if not args.no_probe:
+        result["device_rates"] = probe_part(host, result)
+    # Implicit relations: 2048 candidates f(x1, x2, y) = 0 with up to 8
+    # constants over N = 1048576 rows, 8 LM iterations, LM control on the
+    # device; a CPU-oracle sample beside it.
+    if not args.no_implicit:
+        result["implicit"] = implicit_part(args, host, workloads)
     print(json.dumps(result), flush=True)
 
 
@@ -346,6 +362,7 @@ def verify_part(args, wl, plans, ks, outs, f64outs, ev, sample):
     warm-started from each arm's constants) with the two verified Pareto
     fronts over (loss, node count)."""
     import math
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from rcsr_hip import host
@@ -386,8 +403,32 @@ def verify_part(args, wl, plans, ks, outs, f64outs, ev, sample):
                 best = scores[j]
         return out
     fm, ff = front(vm), front(vf)
-    same = len(fm) == len(ff) and all(a[0] == b[0] and a[1] == b[1] and
-                                      abs(a[2] - b[2]) <= 1e-6 * abs(b[2]) for a, b in zip(fm, ff))
+    # Identical fronts: the same (node count, verified loss) points to 1e-6;
+    # two candidates that tie on both may swap places.
+    same = len(fm) == len(ff) and all(a[1] == b[1] and abs(a[2] - b[2]) <= 1e-6 * abs(b[2])
+                                      for a, b in zip(fm, ff))
+    same_members = same and all(a[0] == b[0] for a, b in zip(fm, ff))
+    # The CPU oracle's own 30-iteration fit on the sample: how far each
+    # device arm's unconverged trajectories are from an independent f64 LM.
+    cpu30, _ = oracle.fit_batch([wl.trees[i] for i in sample], wl.x, wl.y, None, args.iters,
+                                threads=threads, kmax=20)
+    cpu30 = np.asarray(cpu30)
+    def vs_cpu(res):
+        rel = np.array([abs(res[i].sse - cpu30[j]) / cpu30[j] if cpu30[j] > 0 else np.nan
+                        for j, i in enumerate(sample)])
+        cv = np.array([r[1] for r in rows]) if rows else np.array([], bool)
+        idx = {r[0]: q for q, r in enumerate(rows)}
+        conv_j = np.array([idx.get(i) is not None and cv[idx[i]] for i in sample])
+        ok = np.isfinite(rel)
+        out = {"median": float(np.median(rel[ok])), "max": float(np.max(rel[ok])),
+               "frac_le_1e-6": float(np.mean(rel[ok] <= 1e-6))}
+        if (ok & conv_j).any():
+            out["converged_max"] = float(np.max(rel[ok & conv_j]))
+        if (ok & ~conv_j).any():
+            out["unconverged_median"] = float(np.median(rel[ok & ~conv_j]))
+            out["unconverged_frac_le_1e-6"] = float(np.mean(rel[ok & ~conv_j] <= 1e-6))
+        return out
+    mixed_cpu, fp64_cpu = vs_cpu(outs), vs_cpu(f64outs)
     fin = np.isfinite(vm) & np.isfinite(vf) & (vf > 0)
     vrel = (vm[fin] - vf[fin]) / vf[fin]
     log(f"C3 verification sample: {len(sample)} candidates, {int(conv.sum())} converged by "
@@ -410,6 +451,8 @@ def verify_part(args, wl, plans, ks, outs, f64outs, ev, sample):
             "min": float(np.min(vrel)), "frac_mixed_worse_gt_1e-6": float(np.mean(vrel > 1e-6))},
         "verified_front_mixed": fm, "verified_front_fp64": ff,
         "verified_fronts_identical": bool(same),
+        "verified_fronts_same_members": bool(same_members),
+        "mixed_vs_cpu_oracle_30": mixed_cpu, "fp64_device_vs_cpu_oracle_30": fp64_cpu,
         "cpu_refit_seconds": tv,
     }
 
@@ -443,6 +486,111 @@ def arms_part(host, workloads, ncand=256):
         log(f"{name} arm: {ncand / min(walls):.0f} candidates/s ({1e3 * min(walls):.1f} ms)")
     ev.close()
     return out
+
+
+def probe_part(host, result):
+    """host.probe_device_rates (row-evals/s over 2,456,306 rows, fits/s at
+    4 constants x 8 LM iterations on 2,456,306 rows, upload bytes/s, launch
+    seconds) beside the reference's anchors, and the route the cost model
+    then picks for this batch's fit stage with the measured CPU rate."""
+    from rcsr_hip import dispatch as dsp
+    t0 = time.perf_counter()
+    rates = host.probe_device_rates()
+    probe_s = time.perf_counter() - t0
+    anchors = dsp.default_device_rates()
+    out = {"probe_seconds": probe_s,
+           "measured": {"r_eval": rates.r_eval, "r_fit": rates.r_fit, "bw": rates.bw,
+                        "launch_s": rates.launch_s},
+           "reference_anchors": {"r_eval": anchors.r_eval, "r_fit": anchors.r_fit,
+                                 "bw": anchors.bw, "launch_s": anchors.launch_s},
+           "fit_ref": {"rows": dsp.FIT_REF_ROWS, "constants": dsp.FIT_REF_CONSTANTS,
+                       "sweeps": dsp.FIT_REF_SWEEPS}}
+    log(f"device rates: r_eval {rates.r_eval:.3e}/s, r_fit {rates.r_fit:.2f}/s, bw "
+        f"{rates.bw:.3e} B/s, launch {rates.launch_s * 1e6:.1f} us ({probe_s:.1f}s)")
+    cfg = result.get("config", {})
+    cpu = result.get("cpu_baseline")
+    if cpu and cfg:
+        rows, k, sweeps = cfg["n_rows"], cfg["k_mean"], cfg["lm_iterations"]
+        # The CPU rate at the reference shape, from the measured per-candidate
+        # rate at this batch's shape (the model's inverse scaling).
+        cpu_fit_ref = cpu["value"] * (rows / dsp.FIT_REF_ROWS) * (k / dsp.FIT_REF_CONSTANTS) * \
+            (sweeps / dsp.FIT_REF_SWEEPS)
+        cal = dsp.with_device_rates(dsp.Calibration(
+            r_eval=anchors.r_eval, r_fit=anchors.r_fit, bw=anchors.bw, launch_s=anchors.launch_s,
+            cpu_eval=1.5e8, cpu_fit=cpu_fit_ref), rates)
+        d = dsp.Dispatcher(cal).plan(dsp.Stage.FIT, dsp.Workload.fit(
+            cfg["candidates"], rows, max(1, int(round(k))), sweeps))
+        out["fit_route_for_this_batch"] = {
+            "route": d.route.value, "reason": d.reason.value,
+            "predicted_device_s": d.cost.fit_s + d.cost.transfer_s + d.cost.launch_s,
+            "predicted_cpu_s": d.cost.cpu_s, "cpu_fit_at_ref_shape": cpu_fit_ref}
+    return out
+
+
+def implicit_part(args, host, workloads):
+    """The implicit-relation fit at its full shape: config5's candidates
+    (2-8 constants) over N = 1048576 rows of (x1, x2, y) on the noisy unit
+    circle, 8 LM iterations of the geometric residual f / |grad_x f| with a
+    central-difference Jacobian; f64 rows resident.  Candidates/s over the
+    whole batch, and the oracle on one candidate per constant count."""
+    import os
+    import numpy as np
+    import oracle
+    wl = workloads.config5(n=1048576, ncand=args.implicit_cands, iters=8)
+    plans = [host.gpu_fit_plan(t) for t in wl.trees]
+    X = wl.x
+    n, v = X.shape
+    ev = host.GpuEvaluator(np.ascontiguousarray(X.T, np.float32).reshape(-1), n, v,
+                           np.zeros(n, np.float32))
+    assert ev.set_fit_data(np.ascontiguousarray(X.T).reshape(-1), np.zeros(n))
+    outs, ms = ev.implicit_fit_batch(plans[:32], 8)
+    if ms < 0:
+        raise RuntimeError("implicit fit failed")
+    t0 = time.perf_counter()
+    outs, ms = ev.implicit_fit_batch(plans, 8)
+    wall = time.perf_counter() - t0
+    if ms < 0:
+        raise RuntimeError("implicit fit failed")
+    ev.close()
+    st = {}
+    for o in outs:
+        st[str(o.status)] = st.get(str(o.status), 0) + 1
+    ncand = len(plans)
+    log(f"implicit: {ncand} candidates x N={n} in {wall:.2f}s (device {ms:.1f} ms) -> "
+        f"{ncand / wall:.1f} candidates/s")
+    res = {"workload": f"implicit: {ncand} candidates (2-8 constants) x N={n} rows (x1, x2, y), "
+                       "8 LM iterations, geometric residual, central-difference Jacobian",
+           "candidates_per_s": ncand / wall, "seconds": wall, "device_ms": ms,
+           "status_counts": st, "rows": "f64 (the Jacobian's steps of 1e-6 (1 + |c|) "
+           "need f64 arithmetic)"}
+    if args.no_cpu or args.implicit_cpu <= 0:
+        return res
+    by_k = {}
+    for i, p in enumerate(plans):
+        by_k.setdefault(p.nconst, i)
+    sample = [by_k[k] for k in sorted(by_k)][:args.implicit_cpu]
+    threads = min(len(sample), max(1, min(16, os.cpu_count() or 1)))
+    t0 = time.perf_counter()
+    cl, _ = oracle.implicit_fit_batch([wl.trees[i] for i in sample], X, None, 8,
+                                      threads=threads, kmax=8)
+    cpu_s = time.perf_counter() - t0
+    rel = []
+    for j, i in enumerate(sample):
+        if np.isfinite(cl[j]) and cl[j] > 0 and np.isfinite(outs[i].sse):
+            rel.append((outs[i].sse - cl[j]) / cl[j])
+    rel = np.asarray(rel)
+    res["cpu_baseline"] = {
+        "value": len(sample) / cpu_s, "unit": "candidates/s", "cores": threads, "kind": "port",
+        "sample": f"{len(sample)} candidates, one per constant count "
+                  f"{sorted(plans[i].nconst for i in sample)}, full N={n}, 8 LM iterations, "
+                  "one candidate per thread"}
+    res["loss_rel_err_vs_cpu_oracle"] = {
+        "compared": int(len(rel)), "median_abs": float(np.median(np.abs(rel))) if len(rel) else None,
+        "max_abs": float(np.max(np.abs(rel))) if len(rel) else None,
+        "signed": [float(r) for r in rel]}
+    log(f"implicit cpu oracle: {len(sample)} candidates on {threads} threads in {cpu_s:.1f}s; "
+        f"device vs oracle median |rel| {res['loss_rel_err_vs_cpu_oracle']['median_abs']}")
+    return res
 
 
 def order_part(host, workloads, ncand=64, ntrials=4):

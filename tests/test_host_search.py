@@ -148,3 +148,19 @@ def test_fit_pending_launch_failure_metric_and_bound():
     rep = search.fit_pending(_StubEv([0]), isl, [(0, 0)], 100_000, 8, 0,
                              dsp.Dispatcher(_cal()), lambda t: t)
     assert rep.fallbacks == [(0, 0, search.FitFallbackCause.TOO_MANY_CONSTANTS)]
+
+
+def test_device_rates_replace_only_usable_device_terms():
+    # dispatch.rs:514-530: each finite positive probe value replaces the
+    # calibration's device term; the CPU terms and hysteresis stay.
+    cal = _cal()
+    got = dsp.with_device_rates(cal, dsp.DeviceRates(r_eval=3e11, r_fit=50.0, bw=2e10,
+                                                     launch_s=4e-6))
+    assert (got.r_eval, got.r_fit, got.bw, got.launch_s) == (3e11, 50.0, 2e10, 4e-6)
+    assert (got.cpu_eval, got.cpu_fit, got.hysteresis) == (cal.cpu_eval, cal.cpu_fit,
+                                                           cal.hysteresis)
+    bad = dsp.with_device_rates(cal, dsp.DeviceRates(r_eval=math.nan, r_fit=0.0, bw=-1.0,
+                                                     launch_s=-1e-6))
+    assert (bad.r_eval, bad.r_fit, bad.bw, bad.launch_s) == (cal.r_eval, cal.r_fit, cal.bw,
+                                                             cal.launch_s)
+    assert dsp.with_device_rates(cal, dsp.DeviceRates(1.0, 1.0, 1.0, 0.0)).launch_s == 0.0

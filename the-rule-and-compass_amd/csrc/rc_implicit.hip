@@ -243,11 +243,12 @@ __global__ __launch_bounds__(64) void im_pass_kernel(
     const ImItem *__restrict__ items, int nitems, const ImCand *__restrict__ cands,
     const uint32_t *__restrict__ rec, const double *__restrict__ cs, int kmax,
     const double *__restrict__ cols64, const double *__restrict__ w64, int n, int nv, int metric,
-    double *__restrict__ part, int pstride, int *__restrict__ bad)
+    double *__restrict__ part, int pstride, int *__restrict__ bad, const int *__restrict__ run)
 {
     const int it_i = blockIdx.x;
     if (it_i >= nitems) return;
     const ImItem it = items[it_i];
+    if (run && !run[it.cand]) return;
     const ImCand d = cands[it.cand];
     const int lane = threadIdx.x;
     const uint32_t *r4 = rec + d.off;
@@ -344,36 +345,213 @@ __global__ __launch_bounds__(64) void im_pass_kernel(
     if (lane == 0) bad[it_i] = f;
 }
 
-/* implicit.rs:215-253: Gaussian elimination with partial pivoting on the
-   damped normal equations. */
-bool im_solve(const std::vector<double> &A, const std::vector<double> &b, double lambda, int k,
-              std::vector<double> &d)
+/* ---- the LM control of the implicit fit, on the device ------------------------
+
+   One wave per candidate; the state lives in HBM between launches.  The
+   launch sequence of a fit is: INIT, an RSS pass at the start constants,
+   BASE, then per LM iteration a JAC pass, AFTER_JAC (the damped solve and
+   the trial), an RSS pass and AFTER_RSS (accept / reject), and FINAL.  The
+   passes run only the items of candidates whose run flag the previous
+   control step set, so no host round trip sits between iterations.
+   Semantics: implicit.rs:277-409 (lambda 1e-3, x6 on a rejected or failed
+   step, /8 (floor 1e-12) on an accepted one, retirement above 1e9, steps
+   clamped to +-1e6, best-so-far constants). */
+struct ImState {
+    double c[IM_KMAX];
+    double trial[IM_KMAX];
+    double base, best, lambda;
+    int active, want, pad0, pad1;
+};
+
+enum { IC_INIT = 0, IC_BASE = 1, IC_AFTER_JAC = 2, IC_AFTER_RSS = 3, IC_FINAL = 4 };
+
+/* implicit.rs:215-253 on the k x (k + 1) augmented matrix M (row stride
+   IM_KMAX + 1): partial pivoting on the first largest magnitude, a pivot
+   below 1e-300 fails, then back substitution into d.  Every product and
+   sum is rounded on its own, as in the reference's solve. */
+__device__ bool im_solve_dev(double *M, int k, double *d)
 {
-    std::vector<std::vector<double>> m((size_t)k, std::vector<double>((size_t)k + 1, 0.0));
-    for (int i = 0; i < k; ++i) {
-        for (int j = 0; j < k; ++j) m[i][j] = A[(size_t)i * k + j];
-        m[i][i] = m[i][i] * (1.0 + lambda) + 1e-12;
-        m[i][k] = b[i];
-    }
+    /* Separate roundings, as the host solve: no fused multiply-adds. */
+#pragma clang fp contract(off)
+    const int S = IM_KMAX + 1;
     for (int col = 0; col < k; ++col) {
         int piv = col;
         for (int r = col + 1; r < k; ++r)
-            if (fabs(m[r][col]) > fabs(m[piv][col])) piv = r;
-        if (fabs(m[piv][col]) < 1e-300) return false;
-        std::swap(m[col], m[piv]);
+            if (fabs(M[r * S + col]) > fabs(M[piv * S + col])) piv = r;
+        if (fabs(M[piv * S + col]) < 1e-300) return false;
+        if (piv != col)
+            for (int q = 0; q <= k; ++q) {
+                const double t = M[col * S + q];
+                M[col * S + q] = M[piv * S + q];
+                M[piv * S + q] = t;
+            }
         for (int r = col + 1; r < k; ++r) {
-            const double factor = m[r][col] / m[col][col];
+            const double factor = M[r * S + col] / M[col * S + col];
             if (factor != 0.0)
-                for (int cc = col; cc <= k; ++cc) m[r][cc] -= factor * m[col][cc];
+                for (int q = col; q <= k; ++q) M[r * S + q] -= factor * M[col * S + q];
         }
     }
-    d.assign((size_t)k, 0.0);
     for (int i = k - 1; i >= 0; --i) {
-        double s = m[i][k];
-        for (int j = i + 1; j < k; ++j) s -= m[i][j] * d[j];
-        d[i] = s / m[i][i];
+        double acc = M[i * S + k];
+        for (int j = i + 1; j < k; ++j) acc -= M[i * S + j] * d[j];
+        d[i] = acc / M[i * S + i];
     }
     return true;
+}
+
+__global__ __launch_bounds__(64) void im_ctrl_kernel(
+    int op, int np, ImState *__restrict__ S, const ImCand *__restrict__ cands,
+    const int *__restrict__ pre, const double *__restrict__ c0, int kmax, int iters,
+    double *__restrict__ cs, int *__restrict__ run, const int *__restrict__ item0,
+    const int *__restrict__ nitem, const double *__restrict__ part, const int *__restrict__ bad,
+    double *__restrict__ outc, double *__restrict__ outsse, int *__restrict__ outstatus)
+{
+#pragma clang fp contract(off)
+    const int cand = blockIdx.x;
+    if (cand >= np) return;
+    const int lane = threadIdx.x;
+    __shared__ ImState st;
+    __shared__ double G[IM_STATS];
+    __shared__ double M[IM_KMAX * (IM_KMAX + 1)];
+    constexpr int SW = (int)(sizeof(ImState) / sizeof(int));
+    static_assert(sizeof(ImState) % sizeof(int) == 0, "ImState is copied as ints");
+    for (int i = lane; i < SW; i += 64) ((int *)&st)[i] = ((const int *)(S + cand))[i];
+    __syncthreads();
+    const int k = cands[cand].k;
+    const bool host_done = pre[cand] >= 0;
+    const bool ran = run[cand] != 0;
+    const int i0 = item0[cand], ni = nitem[cand];
+    if (op == IC_AFTER_JAC && !host_done && st.active && ran) {
+        const int nst = (k + 1) * (k + 2) / 2;
+        for (int t = lane; t < nst; t += 64) {
+            double acc = 0.0;
+            for (int u = 0; u < ni; ++u) acc += part[(size_t)(i0 + u) * IM_STATS + t];
+            G[t] = acc;
+        }
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double *cc = cs + (size_t)cand * (size_t)kmax;
+        auto flag_sum = [&](double &sum) {
+            int f = 0;
+            double acc = 0.0;
+            for (int u = 0; u < ni; ++u) {
+                f |= bad[i0 + u];
+                acc += part[(size_t)(i0 + u) * IM_STATS];
+            }
+            sum = acc;
+            return f;
+        };
+        int next_run = 0;
+        switch (op) {
+            case IC_INIT: {
+                for (int j = 0; j < IM_KMAX; ++j) st.c[j] = j < kmax ? c0[(size_t)cand * kmax + j] : 0.0;
+                st.base = st.best = __builtin_inf();
+                st.lambda = 1e-3;
+                st.active = 0;
+                st.want = 0;
+                for (int j = 0; j < kmax; ++j) cc[j] = st.c[j];
+                next_run = !host_done;
+                break;
+            }
+            case IC_BASE: {
+                if (host_done || !ran) break;
+                double sum;
+                const int f = flag_sum(sum);
+                const bool ok = f == 0 && isfinite(sum);
+                st.base = st.best = ok ? sum : __builtin_inf();
+                st.active = ok && k > 0 && iters > 0;
+                next_run = st.active;
+                break;
+            }
+            case IC_AFTER_JAC: {
+                st.want = 0;
+                if (host_done || !st.active || !ran) break;
+                int f = 0;
+                for (int u = 0; u < ni; ++u) f |= bad[i0 + u];
+                if (f != 0) {
+                    st.active = 0;
+                    break;
+                }
+                const int Sx = IM_KMAX + 1;
+                bool fin = true;
+                for (int a = 0; a < k; ++a) {
+                    for (int b = 0; b <= a; ++b) {
+                        const double v = G[a * (a + 1) / 2 + b];
+                        M[a * Sx + b] = v;
+                        M[b * Sx + a] = v;
+                        fin = fin && isfinite(v);
+                    }
+                    M[a * Sx + k] = -G[k * (k + 1) / 2 + a];
+                    fin = fin && isfinite(M[a * Sx + k]);
+                }
+                if (!fin) {
+                    st.active = 0;
+                    break;
+                }
+                for (int a = 0; a < k; ++a) M[a * Sx + a] = M[a * Sx + a] * (1.0 + st.lambda) + 1e-12;
+                double step[IM_KMAX];
+                if (!im_solve_dev(M, k, step)) {
+                    st.lambda *= 6.0;
+                    if (st.lambda > 1e9) st.active = 0;
+                    break;
+                }
+                for (int j = 0; j < kmax; ++j) {
+                    const double t = j < k ? fmin(1e6, fmax(-1e6, st.c[j] + step[j])) : st.c[j];
+                    st.trial[j] = t;
+                    cc[j] = t;
+                }
+                st.want = 1;
+                next_run = 1;
+                break;
+            }
+            case IC_AFTER_RSS: {
+                if (host_done || !st.want || !ran) {
+                    next_run = !host_done && st.active;
+                    break;
+                }
+                double sum;
+                const int f = flag_sum(sum);
+                const double t = f == 0 ? sum : __builtin_inf();
+                if (f == 0 && t < st.best) {
+                    st.best = t;
+                    for (int j = 0; j < k; ++j) st.c[j] = st.trial[j];
+                    st.lambda = fmax(st.lambda / 8.0, 1e-12);
+                    if (st.best == 0.0) st.active = 0;
+                } else {
+                    st.lambda *= 6.0;
+                    if (st.lambda > 1e9) st.active = 0;
+                }
+                st.want = 0;
+                for (int j = 0; j < k; ++j) cc[j] = st.c[j];
+                next_run = st.active;
+                break;
+            }
+            case IC_FINAL: {
+                double *oc = outc + (size_t)cand * kmax;
+                const double *sc = c0 + (size_t)cand * kmax;
+                if (host_done) {
+                    for (int j = 0; j < kmax; ++j) oc[j] = sc[j];
+                    outsse[cand] = __builtin_nan("");
+                    outstatus[cand] = 1;
+                } else if (!isfinite(st.base)) {
+                    for (int j = 0; j < kmax; ++j) oc[j] = sc[j];
+                    outsse[cand] = __builtin_inf();
+                    outstatus[cand] = 7;
+                } else {
+                    const bool moved = st.best < st.base;
+                    for (int j = 0; j < kmax; ++j) oc[j] = (moved && j < k) ? st.c[j] : sc[j];
+                    outsse[cand] = st.best;
+                    outstatus[cand] = 0;
+                }
+                break;
+            }
+            default: break;
+        }
+        run[cand] = next_run;
+    }
+    __syncthreads();
+    for (int i = lane; i < SW; i += 64) ((int *)(S + cand))[i] = ((const int *)&st)[i];
 }
 
 /* Host side: validated candidates, resident buffers, pass launcher. */
@@ -507,13 +685,13 @@ struct ImBatch {
         (void)hipEventRecord(c->t0, s);
         if (mode == IM_SCORE)
             im_pass_kernel<IM_SCORE><<<ni, 64, 0, s>>>(d_items, ni, d_cand, d_rec, d_cs, kmax, R.cols64,
-                                                        R.w64, R.n, R.v, metric, d_part, IM_STATS, d_bad);
+                                                        R.w64, R.n, R.v, metric, d_part, IM_STATS, d_bad, nullptr);
         else if (mode == IM_RSS)
             im_pass_kernel<IM_RSS><<<ni, 64, 0, s>>>(d_items, ni, d_cand, d_rec, d_cs, kmax, R.cols64,
-                                                      R.w64, R.n, R.v, metric, d_part, IM_STATS, d_bad);
+                                                      R.w64, R.n, R.v, metric, d_part, IM_STATS, d_bad, nullptr);
         else
             im_pass_kernel<IM_JAC><<<ni, 64, 0, s>>>(d_items, ni, d_cand, d_rec, d_cs, kmax, R.cols64,
-                                                      R.w64, R.n, R.v, metric, d_part, IM_STATS, d_bad);
+                                                      R.w64, R.n, R.v, metric, d_part, IM_STATS, d_bad, nullptr);
         (void)hipEventRecord(c->t1, s);
         if (hipGetLastError() != hipSuccess) return -1;
         if (hipMemcpyAsync(h_part.data(), d_part, sizeof(double) * (size_t)ni * IM_STATS,
@@ -589,94 +767,87 @@ float rc_implicit_fit_batch(void *ctx, const unsigned int *prog, const unsigned 
     ImBatch B;
     if (B.setup(cx, prog, off, pcount, nprog, kmax) != 0) return -1.0f;
     const size_t P = (size_t)nprog, KM = (size_t)kmax;
-    std::vector<double> c(c0, c0 + P * KM), best_c = c, cs = c;
-    std::vector<double> best(P, IM_INF), base(P, IM_INF), lambda(P, 1e-3);
-    std::vector<char> active(P, 0), want(P, 0);
-    std::vector<std::vector<double>> sums;
-    std::vector<int> flags;
-    /* Residual cost at the start (rss_of). */
-    for (size_t i = 0; i < P; ++i) want[i] = B.pre[i] < 0;
-    if (B.pass(IM_RSS, 0, c, want, sums, flags) != 0) return -1.0f;
-    for (size_t i = 0; i < P; ++i) {
-        if (!want[i]) continue;
-        const bool ok = flags[i] == 0 && !sums[i].empty() && isfinite(sums[i][0]);
-        base[i] = ok ? sums[i][0] : IM_INF;
-        best[i] = base[i];
-        /* no constants or no iterations: the start is the answer */
-        active[i] = ok && B.cands[i].k > 0 && iters > 0;
+    /* Each candidate's items are consecutive in row order. */
+    std::vector<int> item0(P, 0), nitem(P, 0);
+    for (size_t q = 0; q < B.h_items.size(); ++q) {
+        const int cd = B.h_items[q].cand;
+        if (nitem[(size_t)cd] == 0) item0[(size_t)cd] = (int)q;
+        nitem[(size_t)cd] += 1;
     }
-    for (int it = 0; it < std::max(iters, 1); ++it) {
-        bool any = false;
-        for (size_t i = 0; i < P; ++i) any = any || active[i];
-        if (!any) break;
-        if (B.pass(IM_JAC, 0, c, active, sums, flags) != 0) return -1.0f;
-        std::fill(want.begin(), want.end(), 0);
-        for (size_t i = 0; i < P; ++i) {
-            if (!active[i]) continue;
-            const int k = B.cands[i].k, K1 = k + 1;
-            const std::vector<double> &t = sums[i];
-            if (flags[i] != 0 || t.empty()) { active[i] = 0; continue; }
-            std::vector<double> jtj((size_t)k * k), rhs((size_t)k);
-            bool fin = true;
-            for (int a = 0; a < k; ++a) {
-                for (int b = 0; b <= a; ++b) {
-                    const double v = t[(size_t)(a * (a + 1) / 2 + b)];
-                    jtj[(size_t)a * k + b] = v;
-                    jtj[(size_t)b * k + a] = v;
-                    fin = fin && isfinite(v);
-                }
-                rhs[a] = -t[(size_t)(k * (k + 1) / 2 + a)];
-                fin = fin && isfinite(rhs[a]);
-            }
-            (void)K1;
-            if (!fin) { active[i] = 0; continue; }
-            std::vector<double> step;
-            if (!im_solve(jtj, rhs, lambda[i], k, step)) {
-                lambda[i] *= 6.0;
-                if (lambda[i] > 1e9) active[i] = 0;
-                continue;
-            }
-            for (int a = 0; a < k; ++a)
-                cs[i * KM + a] = std::min(1e6, std::max(-1e6, c[i * KM + a] + step[a]));
-            want[i] = 1;
-        }
-        bool trial = false;
-        for (size_t i = 0; i < P; ++i) trial = trial || want[i];
-        if (!trial) continue;
-        if (B.pass(IM_RSS, 0, cs, want, sums, flags) != 0) return -1.0f;
-        for (size_t i = 0; i < P; ++i) {
-            if (!want[i]) continue;
-            const bool ok = flags[i] == 0 && !sums[i].empty();
-            const double s = ok ? sums[i][0] : IM_INF;
-            if (ok && s < best[i]) {
-                best[i] = s;
-                for (size_t a = 0; a < KM; ++a) c[i * KM + a] = best_c[i * KM + a] = cs[i * KM + a];
-                lambda[i] = std::max(lambda[i] / 8.0, 1e-12);
-                if (best[i] == 0.0) active[i] = 0;
-            } else {
-                lambda[i] *= 6.0;
-                if (lambda[i] > 1e9) active[i] = 0;
-            }
-        }
+    ImState *d_state = nullptr;
+    int *d_run = nullptr, *d_pre = nullptr, *d_i0 = nullptr, *d_ni = nullptr, *d_st = nullptr;
+    double *d_c0 = nullptr, *d_outc = nullptr, *d_sse = nullptr;
+    auto release = [&]() {
+        if (d_state) (void)hipFree(d_state);
+        if (d_run) (void)hipFree(d_run);
+        if (d_pre) (void)hipFree(d_pre);
+        if (d_i0) (void)hipFree(d_i0);
+        if (d_ni) (void)hipFree(d_ni);
+        if (d_st) (void)hipFree(d_st);
+        if (d_c0) (void)hipFree(d_c0);
+        if (d_outc) (void)hipFree(d_outc);
+        if (d_sse) (void)hipFree(d_sse);
+    };
+    if (hipMalloc((void **)&d_state, sizeof(ImState) * P) != hipSuccess ||
+        hipMalloc((void **)&d_run, sizeof(int) * P) != hipSuccess ||
+        hipMalloc((void **)&d_pre, sizeof(int) * P) != hipSuccess ||
+        hipMalloc((void **)&d_i0, sizeof(int) * P) != hipSuccess ||
+        hipMalloc((void **)&d_ni, sizeof(int) * P) != hipSuccess ||
+        hipMalloc((void **)&d_st, sizeof(int) * P) != hipSuccess ||
+        hipMalloc((void **)&d_c0, sizeof(double) * P * KM) != hipSuccess ||
+        hipMalloc((void **)&d_outc, sizeof(double) * P * KM) != hipSuccess ||
+        hipMalloc((void **)&d_sse, sizeof(double) * P) != hipSuccess) {
+        (void)hipGetLastError();
+        release();
+        return -1.0f;
     }
-    for (size_t i = 0; i < P; ++i) {
-        for (size_t a = 0; a < KM; ++a) outc[i * KM + a] = c0[i * KM + a];
-        if (B.pre[i] >= 0) {
-            outsse[i] = RC_F64_NAN;
-            outstatus[i] = 1;
-            continue;
+    hipStream_t s = cx->s_fit;
+    RcRows &R = cx->sets[0];
+    const int ni = (int)B.h_items.size();
+    bool ok = hipMemcpyAsync(d_pre, B.pre.data(), sizeof(int) * P, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(d_i0, item0.data(), sizeof(int) * P, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(d_ni, nitem.data(), sizeof(int) * P, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(d_c0, c0, sizeof(double) * P * KM, hipMemcpyHostToDevice, s) == hipSuccess;
+    auto ctrl = [&](int op) {
+        im_ctrl_kernel<<<nprog, 64, 0, s>>>(op, nprog, d_state, B.d_cand, d_pre, d_c0, kmax, iters,
+                                            B.d_cs, d_run, d_i0, d_ni, B.d_part, B.d_bad, d_outc,
+                                            d_sse, d_st);
+    };
+    auto pass = [&](int mode) {
+        if (ni <= 0) return;
+        if (mode == IM_RSS)
+            im_pass_kernel<IM_RSS><<<ni, 64, 0, s>>>(B.d_items, ni, B.d_cand, B.d_rec, B.d_cs, kmax,
+                                                      R.cols64, R.w64, R.n, R.v, 0, B.d_part,
+                                                      IM_STATS, B.d_bad, d_run);
+        else
+            im_pass_kernel<IM_JAC><<<ni, 64, 0, s>>>(B.d_items, ni, B.d_cand, B.d_rec, B.d_cs, kmax,
+                                                      R.cols64, R.w64, R.n, R.v, 0, B.d_part,
+                                                      IM_STATS, B.d_bad, d_run);
+    };
+    if (ok) {
+        (void)hipEventRecord(cx->t0, s);
+        ctrl(IC_INIT);
+        pass(IM_RSS);
+        ctrl(IC_BASE);
+        for (int it = 0; it < std::max(iters, 1); ++it) {
+            pass(IM_JAC);
+            ctrl(IC_AFTER_JAC);
+            pass(IM_RSS);
+            ctrl(IC_AFTER_RSS);
         }
-        if (!isfinite(base[i])) {
-            outsse[i] = IM_INF;
-            outstatus[i] = 7;
-            continue;
-        }
-        if (best[i] < base[i])
-            for (size_t a = 0; a < KM; ++a) outc[i * KM + a] = best_c[i * KM + a];
-        outsse[i] = best[i];
-        outstatus[i] = 0;
+        ctrl(IC_FINAL);
+        (void)hipEventRecord(cx->t1, s);
+        ok = hipGetLastError() == hipSuccess &&
+             hipMemcpyAsync(outc, d_outc, sizeof(double) * P * KM, hipMemcpyDeviceToHost, s) == hipSuccess &&
+             hipMemcpyAsync(outsse, d_sse, sizeof(double) * P, hipMemcpyDeviceToHost, s) == hipSuccess &&
+             hipMemcpyAsync(outstatus, d_st, sizeof(int) * P, hipMemcpyDeviceToHost, s) == hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
     }
-    return B.ms;
+    float ms = -1.0f;
+    if (ok) (void)hipEventElapsedTime(&ms, cx->t0, cx->t1);
+    (void)hipGetLastError();
+    release();
+    return ok ? ms : -1.0f;
 }
 
 } /* extern "C" */

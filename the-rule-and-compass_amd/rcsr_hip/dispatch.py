@@ -23,6 +23,12 @@ from typing import Dict, Optional
 FIT_REF_ROWS = 2_456_306.0
 FIT_REF_CONSTANTS = 4.0
 FIT_REF_SWEEPS = 8.0
+# The reference's device anchors (dispatch.rs:69-81): row-evals/s, accepted
+# fits/s at the reference shape, host-device bytes/s and launch seconds.
+DEFAULT_R_EVAL = 20.0e9
+DEFAULT_R_FIT = 1.0 / 0.0895
+DEFAULT_BW = 12.0e9
+DEFAULT_LAUNCH_S = 5.0e-6
 
 
 class Stage(Enum):
@@ -132,6 +138,37 @@ class Calibration:
     def cpu_fit_rate(self, rows: int, constants: int, sweeps: int) -> float:
         """Exact CPU fits per second at the same shape."""
         return self._scale(self.cpu_fit, rows, constants, sweeps)
+
+
+@dataclass
+class DeviceRates:
+    """dispatch.rs:642-660: device rates measured by a hardware probe:
+    row-evals/s, fits/s at the reference shape (FIT_REF_ROWS rows,
+    FIT_REF_CONSTANTS constants, FIT_REF_SWEEPS sweeps), host-device bytes/s
+    and the launch overhead in seconds."""
+    r_eval: float
+    r_fit: float
+    bw: float
+    launch_s: float
+
+
+def default_device_rates() -> DeviceRates:
+    """dispatch.rs:649-658: the anchors, for a run without a probe."""
+    return DeviceRates(DEFAULT_R_EVAL, DEFAULT_R_FIT, DEFAULT_BW, DEFAULT_LAUNCH_S)
+
+
+def with_device_rates(cal: "Calibration", rates: DeviceRates) -> "Calibration":
+    """dispatch.rs:514-530 Calibration::with_device_rates: each usable probe
+    value replaces the calibration's device term; an unusable one (not
+    finite, or not positive; a negative launch time) leaves it."""
+    def usable(v, allow_zero=False):
+        return math.isfinite(v) and (v >= 0.0 if allow_zero else v > 0.0)
+    return Calibration(
+        r_eval=rates.r_eval if usable(rates.r_eval) else cal.r_eval,
+        r_fit=rates.r_fit if usable(rates.r_fit) else cal.r_fit,
+        bw=rates.bw if usable(rates.bw) else cal.bw,
+        launch_s=rates.launch_s if usable(rates.launch_s, True) else cal.launch_s,
+        cpu_eval=cal.cpu_eval, cpu_fit=cal.cpu_fit, hysteresis=cal.hysteresis)
 
 
 @dataclass

@@ -466,6 +466,87 @@ class GpuEvaluator:
         return name.decode() if name else "unknown"
 
 
+def probe_device_rates(rows: int = 2456306, fits: int = 64, programs: int = 256,
+                       seed: int = 11):
+    """The hardware probe behind dispatch.DeviceRates (dispatch.rs:642-660),
+    measured on the current device with synthetic rows of three inputs:
+
+      r_eval    row-evals/s of a batch of `programs` programs (21-31 nodes)
+                over `rows` rows, from the batch's device time;
+      r_fit     accepted fits/s at the reference shape (FIT_REF_ROWS rows,
+                FIT_REF_CONSTANTS = 4 constants, FIT_REF_SWEEPS = 8 LM
+                iterations): `fits` candidates in one batch, wall clock;
+      bw        host-to-device bytes/s of re-uploading the f64 fit rows;
+      launch_s  the median wall time of a one-program batch over 64 rows.
+
+    Raises RuntimeError when the device is missing or a call fails."""
+    import time
+    from . import dispatch
+    from .expr import ADD, COS, MUL, SIN, const, node, var
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-2.0, 2.0, size=(rows, 3))
+    y = 1.5 * x[:, 0] - 0.7 * np.sin(x[:, 1]) + 0.3 * x[:, 0] * x[:, 2] + 0.2
+    cols32 = np.ascontiguousarray(x.T, np.float32).reshape(-1)
+    cols64 = np.ascontiguousarray(x.T).reshape(-1)
+    ev = GpuEvaluator(cols32, rows, 3, y.astype(np.float32))
+    try:
+        # r_eval: programs of 21-31 nodes.
+        pop = Population()
+        while len(pop) < programs:
+            t = var(int(rng.integers(0, 3)))
+            for _ in range(int(rng.integers(5, 8))):
+                u = node(SIN if rng.random() < 0.5 else COS, var(int(rng.integers(0, 3))))
+                t = node(ADD, t, node(MUL, const(float(rng.normal())), u))
+            pop.add(t)
+        prog, off = pop.arrays()
+        out, ms = ev.eval_words(prog, off, 0)
+        out, ms = ev.eval_words(prog, off, 0)
+        if out is None or ms <= 0:
+            raise RuntimeError("evaluation probe failed")
+        r_eval = len(pop) * rows / (ms * 1e-3)
+        # bw: the second upload of the f64 rows reuses the device buffers.
+        if not ev.set_fit_data(cols64, y):
+            raise RuntimeError("set_fit_data failed")
+        t0 = time.perf_counter()
+        if not ev.set_fit_data(cols64, y):
+            raise RuntimeError("set_fit_data failed")
+        bw = (cols64.nbytes + y.nbytes) / (time.perf_counter() - t0)
+        # r_fit: 4-constant candidates, 8 LM iterations.
+        plans = []
+        for _ in range(fits):
+            c = rng.normal(size=4)
+            t = node(ADD, node(ADD, node(MUL, const(c[0]), var(0)),
+                               node(MUL, const(c[1]), node(SIN, var(1)))),
+                     node(ADD, node(MUL, const(c[2]), node(MUL, var(0), var(2))), const(c[3])))
+            plans.append(gpu_fit_plan(t))
+        outs, ms = ev.fit_batch(plans, 8, 0)
+        t0 = time.perf_counter()
+        outs, ms = ev.fit_batch(plans, 8, 0)
+        wall = time.perf_counter() - t0
+        if ms < 0:
+            raise RuntimeError("fit probe failed")
+        r_fit = fits / wall
+    finally:
+        ev.close()
+    small = GpuEvaluator(np.ascontiguousarray(x[:64].T, np.float32).reshape(-1), 64, 3,
+                         y[:64].astype(np.float32))
+    try:
+        one = Population()
+        one.add(node(ADD, var(0), var(1)))
+        p1, o1 = one.arrays()
+        walls = []
+        for _ in range(21):
+            t0 = time.perf_counter()
+            out, ms = small.eval_words(p1, o1, 0)
+            walls.append(time.perf_counter() - t0)
+            if out is None or ms < 0:
+                raise RuntimeError("launch probe failed")
+        launch_s = float(np.median(walls[1:]))
+    finally:
+        small.close()
+    return dispatch.DeviceRates(r_eval=r_eval, r_fit=r_fit, bw=bw, launch_s=launch_s)
+
+
 class BatchStatus(Enum):
     """fit/gpu.rs:61-93."""
     GPU = "gpu"
