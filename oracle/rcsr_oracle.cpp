@@ -12,10 +12,18 @@
  *   fit_lm / flatten / solve / sse / normal_equations: pinned bit-exactly by
  *     the six recorded golden cases (fit.rs:4654-4832).
  *   backward_impl: pinned by central differences and closed-form derivatives.
- *   irls_fit: pinned by the weighted-median closed form, exact recovery of a
- *     clean law and the never-worse-than-start invariant.
+ *   irls_fit (L1): pinned by the weighted-median closed form, exact recovery
+ *     of a clean law and the never-worse-than-start invariant.
+ *   irls_fit (log-squashed): pinned by scipy's minimum of the same objective,
+ *     by beating the squared fit on its own objective (fit.rs:4506) and by
+ *     the base weight reaching the IRLS weight.
+ *   metric_score: pinned by numpy means over the rows (squared, absolute,
+ *     log-squashed; unweighted, weighted with zero weights, a weight of two
+ *     as a repeated row) and the weight and finiteness rules.
  *   erf/erfc: pinned against the C library's erf/erfc (1e-14).
- *   implicit: pinned by the unit-circle closed form and scale invariance.
+ *   implicit: pinned by the unit-circle closed form and scale invariance;
+ *     implicit_score by a numpy weighted mean of the distances and the
+ *     undefined-row rules.
  */
 #include "rcsr_oracle.h"
 
