@@ -69,9 +69,13 @@ template <> struct VmAsmShape<4> {
    the loss sum, bad-row and skipped-row flags), 64 R rows per wave step.
    The streams hold the inputs in U slots 0..v-1.  For KC = 32 only the
    K <= 22 live columns are stored; the MFMA reads of the padding columns
-   are clamped to the last stored one and land only in discarded entries. */
+   are clamped to the last stored one and land only in discarded entries.
+   The LDS block allows at most two waves per SIMD.  The polish variant,
+   whose f64 residual sweep needs more registers than two waves hold, is
+   still held to two (it spills a few hundred bytes per lane, and measured
+   1.9 ms faster per C3 fit than at one wave, with the same bits). */
 template <int KC, bool POLISH, int R>
-__global__ __launch_bounds__(64) void vm_asm_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(POLISH ? 2 : 1))) void vm_asm_kernel(
     const FtItem *__restrict__ items, const int *__restrict__ list, int nlist,
     const FtCand *__restrict__ cands, const FtRows *__restrict__ rowsd,
     const uint32_t *__restrict__ code, const int *__restrict__ run, const float *__restrict__ cp32,
