@@ -5,9 +5,12 @@ through the C ABI.
 The reference scores and fits implicit relations on the CPU only
 (implicit.rs:59-411); the oracle restates it in f64 and is pinned by the
 exact unit-circle distance.  The device arm computes the same f64 distance
-f / ||grad_x f|| with an adjoint input-gradient sweep and the same
-central-difference LM, so scores are compared at relative 1e-9 and fitted
-losses at 1e-6.  The f32 adjoint gradients (the mixed-precision fit's
+f / ||grad_x f|| and the same central-difference LM, so scores are compared
+at relative 1e-9 and fitted losses at 1e-6.  Programs whose evaluation
+stack stays within 9 values over at most 4 inputs run a register-stack
+sweep with forward-mode input tangents; deeper ones run the node-array
+sweep with an adjoint input gradient; both are held to the oracle, and to
+each other on the same relation written both ways.  The f32 adjoint gradients (the mixed-precision fit's
 Jacobian and the input gradient) are checked against central differences of
 the f64 oracle.
 """
@@ -64,6 +67,53 @@ def test_implicit_scores_match_the_oracle():
                 assert st[i] == 0
                 assert out[i] == pytest.approx(want, rel=1e-9), (metric, i)
     ev.close()
+
+
+def _right_deep(terms):
+    tree = terms[-1]
+    for t in reversed(terms[:-1]):
+        tree = node(ADD, t, tree)
+    return tree
+
+
+def _left_deep(terms):
+    tree = terms[0]
+    for t in terms[1:]:
+        tree = node(ADD, tree, t)
+    return tree
+
+
+def test_register_stack_and_node_array_sweeps_agree():
+    # The same relation as a left-deep sum (stack depth 3: the register
+    # sweep) and as a right-deep sum of 12 terms (depth 13: the node-array
+    # sweep), scored and fitted on the device against the oracle.
+    X, _ = _circle(4096)
+    def terms(c):
+        basis = [node(SQUARE, var(0)), node(SQUARE, var(1)), node(MUL, var(0), var(1)), var(0),
+                 var(1), var(2), node(SIN, var(0)), node(COS, var(1)), node(EXP, var(2)),
+                 node(MUL, var(2), var(0)), node(SQUARE, var(2)), node(MUL, var(1), var(2))]
+        return [node(MUL, const(c[j]), basis[j]) for j in range(len(basis))]
+    c = [1.0, 1.05, 0.02, -0.01, 0.015, 0.01, 0.005, -0.004, 0.001, 0.003, -0.002, 0.001]
+    left = node(SUB, _left_deep(terms(c)), const(1.0))
+    right = node(SUB, _right_deep(terms(c)), const(1.0))
+    plans = [host.gpu_fit_plan(t) for t in (left, right)]
+    assert all(p is not None for p in plans)
+    ev = _evaluator(X)
+    for metric in (0, 1):
+        out, st, ms = ev.implicit_score_batch(plans, metric)
+        assert ms >= 0 and list(st) == [0, 0]
+        for t, o in zip((left, right), out):
+            assert o == pytest.approx(oracle.implicit_score(t, X, None, metric), rel=1e-9)
+        assert out[0] == pytest.approx(out[1], rel=1e-9)
+    outs, ms = ev.implicit_fit_batch(plans, 4)
+    ev.close()
+    assert ms >= 0
+    want_l, _ = oracle.implicit_fit_batch([left, right], X, None, 4, threads=2, kmax=13)
+    for o, wl in zip(outs, want_l):
+        _say(f"sweeps: device {o.sse:.10e} oracle {wl:.10e} status {o.status}")
+        assert o.status == 0
+        assert o.sse <= wl * (1 + 1e-4)
+    assert outs[0].sse == pytest.approx(outs[1].sse, rel=1e-4)
 
 
 def test_implicit_score_weights_follow_the_reference():

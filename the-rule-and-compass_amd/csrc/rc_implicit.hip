@@ -52,8 +52,17 @@ struct ImCand {
     uint32_t off; /* first record word */
     int m;        /* nodes */
     int k;        /* constants */
-    int pad;
+    int fast;     /* 1: runs the register-stack sweep (im_fast_kernel) */
 };
+
+/* The register-stack sweep's limits: values below the top of the stack
+   (the post-order evaluation's depth less one) and input columns. */
+constexpr int IM_SDEPTH = 8;
+constexpr int IM_NVMAX = 4;
+/* A stack file: two components of the 8 stack entries, one in elements
+   0..7 and one in 8..15 (a 16-element vector is indexed with one register
+   move per access; shorter ones are lowered to select chains). */
+typedef double im_d16 __attribute__((ext_vector_type(16)));
 
 struct ImItem {
     int cand;
@@ -250,6 +259,7 @@ __global__ __launch_bounds__(64) void im_pass_kernel(
     const ImItem it = items[it_i];
     if (run && !run[it.cand]) return;
     const ImCand d = cands[it.cand];
+    if (d.fast) return;
     const int lane = threadIdx.x;
     const uint32_t *r4 = rec + d.off;
     const double *c = cs + (size_t)it.cand * (size_t)kmax;
@@ -320,8 +330,323 @@ __global__ __launch_bounds__(64) void im_pass_kernel(
                 while ((a + 1) * (a + 2) / 2 <= s) ++a;
                 const int b = s - a * (a + 1) / 2;
                 double t = acc[q];
+                /* Rolled by 8: a full unroll hoists all 128 LDS reads and
+                   spills. */
+#pragma unroll 8
                 for (int r = 0; r < 64; ++r) t += J[r][a] * J[r][b];
                 acc[q] = t;
+            }
+            __syncthreads();
+        }
+    }
+    double *p = part + (size_t)it_i * (size_t)pstride;
+    int f = flag;
+    for (int off = 32; off > 0; off >>= 1) f |= __shfl_xor(f, off, 64);
+    if (MODE == IM_JAC) {
+        for (int q = 0; q < 4; ++q) {
+            const int s = lane + 64 * q;
+            if (s < nstats) p[s] = acc[q];
+        }
+    } else {
+        const double t0 = rc_wave_sum_f64(s0);
+        const double t1 = rc_wave_sum_f64(s1);
+        if (lane == 0) {
+            p[0] = t0;
+            p[1] = t1;
+        }
+    }
+    if (lane == 0) bad[it_i] = f;
+}
+
+/* ---- the register-stack sweep ------------------------------------------------
+
+   The same per-row contract as im_distance for programs whose post-order
+   evaluation keeps at most IM_SDEPTH values below the top of its stack, over
+   at most IM_NVMAX input columns.  One forward sweep carries, per stack
+   entry, the value_of value, the guarded (expr::eval) value and the
+   tangents d/dx_q of the value_of value (forward-mode differentiation)
+   instead of a value sweep, a guarded re-evaluation and an adjoint sweep
+   over per-lane node arrays.  The stack lives in registers: the top in
+   fixed registers, the entries below it in vectors indexed by the
+   wave-uniform stack pointer.
+
+   Every local partial is the adjoint sweep's (implicit.rs via the fitter's
+   backward rules), applied to the tangents with the adjoint sweep's zero
+   rule: a zero tangent or a zero partial contributes exactly 0, as a zero
+   adjoint skips a node there.  The gradient therefore agrees with the
+   adjoint sweep's up to the rounding of the products' association; where
+   an exactly cancelling pair of adjoints (which the adjoint sweep skips)
+   sits above an infinite partial, the two may differ, and both then flag
+   the row in every case the tests hold. */
+__device__ __forceinline__ double im_lm(double p, double t)
+{
+    return (p == 0.0 || t == 0.0) ? 0.0 : p * t;
+}
+__device__ __forceinline__ double im_ld(double t, double x) { return t == 0.0 ? 0.0 : t / x; }
+
+/* A unary node of value_of argument a and guarded argument ga: its value,
+   guarded value, whether a guard fires, and its local partial p (the
+   tangent is t / p when div, else p * t, both under the zero rule).  Out
+   of line, so the sweep's loop keeps few registers. */
+struct ImUnary {
+    double v, g, p;
+    int div, guard;
+};
+__device__ __noinline__ ImUnary im_unary_all(uint32_t op, double a, double ga)
+{
+    ImUnary u;
+    u.v = im_value(op, a, 0.0);
+    u.g = im_guarded(op, ga, 0.0);
+    u.guard = im_guard(op, a, 0.0, u.v) ? 1 : 0;
+    u.div = 0;
+    switch (op) {
+        case OP_NEG: u.p = -1.0; break;
+        case OP_EXP: u.p = exp(a); break;
+        case OP_LOG: u.p = a; u.div = 1; break;
+        case OP_ABS: u.p = a >= 0.0 ? 1.0 : -1.0; break;
+        case OP_SQRT: u.p = 2.0 * sqrt(a); u.div = 1; break;
+        case OP_SIN: u.p = cos(a); break;
+        case OP_COS: u.p = -sin(a); break;
+        case OP_TAN: { const double c = cos(a); u.p = c * c; u.div = 1; } break;
+        case OP_SINH: u.p = cosh(a); break;
+        case OP_COSH: u.p = sinh(a); break;
+        case OP_TANH: { const double h = tanh(a); u.p = 1.0 - h * h; } break;
+        case OP_SIG: { const double e = 1.0 / (1.0 + exp(-a)); u.p = e * (1.0 - e); } break;
+        case OP_GAUSS: u.p = (-2.0 * a) * exp(-a * a); break;
+        case OP_SQUARE: u.p = 2.0 * a; break;
+        case OP_CUBE: u.p = 3.0 * a * a; break;
+        case OP_ERF: u.p = 2.0 * IM_INV_SQRT_PI * exp(-a * a); break;
+        case OP_ERFC: u.p = -2.0 * IM_INV_SQRT_PI * exp(-a * a); break;
+        default: u.p = 0.0; break;
+    }
+    return u;
+}
+
+/* A POW node: value, guarded value, guard, and the partials of both
+   arguments (zero unless the base is positive). */
+struct ImPow {
+    double v, g, pa, pb;
+    int guard;
+};
+__device__ __noinline__ ImPow im_pow_all(double a, double b, double ga, double gb)
+{
+    ImPow w;
+    w.v = im_value(OP_POW, a, b);
+    w.g = im_guarded(OP_POW, ga, gb);
+    w.guard = im_guard(OP_POW, a, b, w.v) ? 1 : 0;
+    if (a > 0.0) {
+        w.pa = b * pow(a, b - 1.0);
+        w.pb = pow(a, b) * log(a);
+    } else {
+        w.pa = 0.0;
+        w.pb = 0.0;
+    }
+    return w;
+}
+
+/* The signed distance (metric 0) or f (metric 1) of one row at the
+   constants c, with constant slot pj replaced by cpj (pj < 0: none); x
+   holds the row's NV inputs.  False when the row is undefined. */
+template <int NV>
+__device__ __forceinline__ bool im_fast_distance(const uint32_t *__restrict__ r4, int m,
+                                                 const double *__restrict__ c, int pj, double cpj,
+                                                 const double (&x)[IM_NVMAX], int metric,
+                                                 double &out)
+{
+    im_d16 FV, F01, F23; /* value | guarded value, t0 | t1, t2 | t3 */
+    double tv = 0.0, tg = 0.0, t[IM_NVMAX] = {0.0, 0.0, 0.0, 0.0};
+    int sp = -1; /* entries below the top; -1: empty */
+    bool guard = false;
+    for (int i = 0; i < m; ++i) {
+        const uint32_t op = r4[4 * i], slot = r4[4 * i + 3];
+        if (op == OP_CONST || op == OP_VAR) {
+            if (sp >= 0) {
+                FV[sp] = tv;
+                FV[sp + 8] = tg;
+                F01[sp] = t[0];
+                if (NV > 1) F01[sp + 8] = t[1];
+                if (NV > 2) F23[sp] = t[2];
+                if (NV > 3) F23[sp + 8] = t[3];
+            }
+            ++sp;
+            if (op == OP_CONST) {
+                tv = ((int)slot == pj) ? cpj : c[slot];
+#pragma unroll
+                for (int q = 0; q < IM_NVMAX; ++q) t[q] = 0.0;
+            } else {
+                tv = slot == 0 ? x[0] : slot == 1 ? x[1] : slot == 2 ? x[2] : x[3];
+#pragma unroll
+                for (int q = 0; q < IM_NVMAX; ++q) t[q] = ((int)slot == q) ? 1.0 : 0.0;
+            }
+            tg = tv;
+        } else if (im_is_bin(op)) {
+            --sp;
+            const double va = FV[sp], ga = FV[sp + 8];
+            double ta[IM_NVMAX];
+            ta[0] = F01[sp];
+            ta[1] = NV > 1 ? F01[sp + 8] : 0.0;
+            ta[2] = NV > 2 ? F23[sp] : 0.0;
+            ta[3] = NV > 3 ? F23[sp + 8] : 0.0;
+            const double vb = tv, gb = tg;
+            if (op == OP_POW) {
+                const ImPow w = im_pow_all(va, vb, ga, gb);
+                guard = guard || w.guard;
+                tg = w.g;
+                tv = w.v;
+#pragma unroll
+                for (int q = 0; q < NV; ++q) t[q] = im_lm(w.pa, ta[q]) + im_lm(w.pb, t[q]);
+            } else {
+                /* ADD, SUB, MUL, DIV: no guard fires; the guarded value is
+                   value_of's on the guarded arguments. */
+                auto arith = [op](double l, double r) {
+                    if (op == OP_ADD) return l + r;
+                    if (op == OP_SUB) return l - r;
+                    if (op == OP_MUL) return l * r;
+                    return fabs(r) < 1e-12 ? RC_F64_NAN : l / r; /* value_of's DIV */
+                };
+                tv = arith(va, vb);
+                tg = arith(ga, gb);
+#pragma unroll
+                for (int q = 0; q < NV; ++q) {
+                    const double a_ = ta[q], b_ = t[q];
+                    double r;
+                    if (op == OP_ADD) r = a_ + b_;
+                    else if (op == OP_SUB) r = a_ - b_;
+                    else if (op == OP_MUL) r = im_lm(vb, a_) + im_lm(va, b_);
+                    else r = im_ld(a_, vb) + (b_ == 0.0 ? 0.0 : -(va * b_) / (vb * vb));
+                    t[q] = r;
+                }
+            }
+        } else {
+            const ImUnary u = im_unary_all(op, tv, tg);
+            guard = guard || u.guard;
+            tg = u.g;
+            tv = u.v;
+#pragma unroll
+            for (int q = 0; q < NV; ++q) t[q] = u.div ? im_ld(t[q], u.p) : im_lm(u.p, t[q]);
+        }
+    }
+    const double fv = guard ? tg : tv;
+    if (!isfinite(fv)) return false;
+    if (metric == 1) {
+        out = fv;
+        return true;
+    }
+    double norm_sq = 0.0;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        if (!isfinite(t[q])) return false;
+        norm_sq += t[q] * t[q];
+    }
+    const double norm = sqrt(norm_sq);
+    if (!(isfinite(norm) && norm > 0.0)) return false;
+    const double dd = fv / norm;
+    if (!isfinite(dd)) return false;
+    out = dd;
+    return true;
+}
+
+/* im_pass_kernel on the register-stack sweep for the candidates marked
+   fast (the general kernel skips them, this one skips the others); NV is
+   the row set's input count. */
+template <int MODE, int NV>
+__global__ __launch_bounds__(64) void im_fast_kernel(
+    const ImItem *__restrict__ items, int nitems, const ImCand *__restrict__ cands,
+    const uint32_t *__restrict__ rec, const double *__restrict__ cs, int kmax,
+    const double *__restrict__ cols64, const double *__restrict__ w64, int n, int metric,
+    double *__restrict__ part, int pstride, int *__restrict__ bad, const int *__restrict__ run)
+{
+    const int it_i = blockIdx.x;
+    if (it_i >= nitems) return;
+    const ImItem it = items[it_i];
+    if (run && !run[it.cand]) return;
+    const ImCand d = cands[it.cand];
+    if (!d.fast) return;
+    const int lane = threadIdx.x;
+    const uint32_t *r4 = rec + d.off;
+    const double *c = cs + (size_t)it.cand * (size_t)kmax;
+    const int k = d.k;
+    __shared__ double J[64][IM_KMAX + 2];
+    const int K1 = k + 1; /* [J | r] columns */
+    const int nstats = K1 * (K1 + 1) / 2;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0}; /* owned Gram entries (JAC) */
+    double s0 = 0.0, s1 = 0.0;
+    int flag = 0;
+    const int row_end = it.row0 + it.rows;
+    for (int base = it.row0; base < row_end; base += 64) {
+        const int row = base + lane;
+        const bool live = row < row_end;
+        const double w = (live && w64) ? w64[row] : 1.0;
+        if (live && (!isfinite(w) || w < 0.0)) flag |= 2;
+        const bool use = live && isfinite(w) && w > 0.0;
+        double x[IM_NVMAX] = {0.0, 0.0, 0.0, 0.0};
+        if (use) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) x[q] = cols64[(size_t)q * (size_t)n + (size_t)row];
+        }
+        if (MODE == IM_SCORE || MODE == IM_RSS) {
+            if (use) {
+                double dist;
+                if (!im_fast_distance<NV>(r4, d.m, c, -1, 0.0, x, MODE == IM_SCORE ? metric : 0,
+                                          dist)) {
+                    flag |= 1;
+                } else if (MODE == IM_SCORE) {
+                    s0 += w * fabs(dist);
+                    s1 += w;
+                } else {
+                    const double r = sqrt(w) * dist;
+                    s0 += r * r;
+                }
+            }
+        } else {
+            for (int j = 0; j <= k; ++j) J[lane][j] = 0.0;
+            if (use) {
+                /* The 2k + 1 distances of the central difference in one
+                   loop (one copy of the sweep): e = 0 at c, then c_j + h
+                   and c_j - h for each j.  A row stops at its first
+                   undefined distance, as the node-array kernel's does. */
+                const double sw = sqrt(w);
+                bool ok = true, fin = true;
+                double dp = 0.0;
+                for (int e = 0; e <= 2 * k && ok; ++e) {
+                    const int j = (e - 1) >> 1;
+                    double cp = 0.0, h = 0.0;
+                    if (e > 0) {
+                        const double cj = c[j];
+                        h = 1e-6 * (1.0 + fabs(cj));
+                        cp = (e & 1) ? cj + h : cj - h;
+                    }
+                    double dist = 0.0;
+                    ok = im_fast_distance<NV>(r4, d.m, c, e > 0 ? j : -1, cp, x, 0, dist);
+                    if (!ok) break;
+                    if (e == 0) {
+                        J[lane][k] = sw * dist;
+                        fin = isfinite(J[lane][k]);
+                    } else if (e & 1) {
+                        dp = dist;
+                    } else {
+                        const double g = (sw * dp - sw * dist) / (2.0 * h);
+                        J[lane][j] = g;
+                        fin = fin && isfinite(g);
+                    }
+                }
+                if (!ok || !fin) flag |= 1;
+            }
+            __syncthreads();
+            for (int q = 0; q < 4; ++q) {
+                const int s = lane + 64 * q;
+                if (s >= nstats) break;
+                /* packed lower triangle: s = a (a + 1) / 2 + b */
+                int a = (int)((sqrt(8.0 * s + 1.0) - 1.0) * 0.5);
+                while (a * (a + 1) / 2 > s) --a;
+                while ((a + 1) * (a + 2) / 2 <= s) ++a;
+                const int b = s - a * (a + 1) / 2;
+                double tt = acc[q];
+                /* Rolled by 8: a full unroll hoists all 128 LDS reads and
+                   spills. */
+#pragma unroll 8
+                for (int r = 0; r < 64; ++r) tt += J[r][a] * J[r][b];
+                acc[q] = tt;
             }
             __syncthreads();
         }
@@ -570,6 +895,28 @@ struct ImBatch {
     std::vector<double> h_part;
     std::vector<int> h_bad;
     float ms = 0.0f;
+    bool any_fast = false, any_general = false;
+
+    /* One pass over ni items of mode MODE on stream s: the register-stack
+       kernel for the fast candidates, the node-array kernel for the rest. */
+    template <int MODE>
+    void launch(hipStream_t s, int ni, int metric, const int *run)
+    {
+        if (ni <= 0) return;
+        RcRows &R = c->sets[0];
+        if (any_fast) {
+            switch (R.v) {
+                case 1: im_fast_kernel<MODE, 1><<<ni, 64, 0, s>>>(d_items, ni, d_cand, d_rec, d_cs, kmax, R.cols64, R.w64, R.n, metric, d_part, IM_STATS, d_bad, run); break;
+                case 2: im_fast_kernel<MODE, 2><<<ni, 64, 0, s>>>(d_items, ni, d_cand, d_rec, d_cs, kmax, R.cols64, R.w64, R.n, metric, d_part, IM_STATS, d_bad, run); break;
+                case 3: im_fast_kernel<MODE, 3><<<ni, 64, 0, s>>>(d_items, ni, d_cand, d_rec, d_cs, kmax, R.cols64, R.w64, R.n, metric, d_part, IM_STATS, d_bad, run); break;
+                default: im_fast_kernel<MODE, 4><<<ni, 64, 0, s>>>(d_items, ni, d_cand, d_rec, d_cs, kmax, R.cols64, R.w64, R.n, metric, d_part, IM_STATS, d_bad, run); break;
+            }
+        }
+        if (any_general)
+            im_pass_kernel<MODE><<<ni, 64, 0, s>>>(d_items, ni, d_cand, d_rec, d_cs, kmax, R.cols64,
+                                                   R.w64, R.n, R.v, metric, d_part, IM_STATS, d_bad,
+                                                   run);
+    }
 
     ~ImBatch()
     {
@@ -600,6 +947,7 @@ struct ImBatch {
             bool ok = words % 4 == 0 && m >= 1 && m <= IM_MAX_NODES && p >= 0 && p <= kmax &&
                       p <= IM_KMAX;
             std::vector<int> st;
+            size_t depth = 0;
             for (int q = 0; ok && q < m; ++q) {
                 const uint32_t op = w[4 * q], k0 = w[4 * q + 1], k1 = w[4 * q + 2], slot = w[4 * q + 3];
                 if (op == OP_CONST) {
@@ -622,6 +970,7 @@ struct ImBatch {
                     ok = known && !st.empty() && st.back() == (int)k0;
                     if (ok) { st.pop_back(); st.push_back(q); }
                 }
+                depth = std::max(depth, st.size());
             }
             ok = ok && st.size() == 1 && st[0] == m - 1;
             if (!ok) {
@@ -631,6 +980,8 @@ struct ImBatch {
             cands[i].off = (uint32_t)rec.size();
             cands[i].m = m;
             cands[i].k = p;
+            cands[i].fast = (depth <= (size_t)IM_SDEPTH + 1 && R.v >= 1 && R.v <= IM_NVMAX) ? 1 : 0;
+            (cands[i].fast ? any_fast : any_general) = true;
             rec.insert(rec.end(), w, w + words);
         }
         if (rec.empty()) rec.push_back(0);
@@ -666,7 +1017,6 @@ struct ImBatch {
     int pass(int mode, int metric, const std::vector<double> &cs, const std::vector<char> &run,
              std::vector<std::vector<double>> &sums, std::vector<int> &flags)
     {
-        RcRows &R = c->sets[0];
         std::vector<ImItem> sel;
         std::vector<int> from;
         for (size_t q = 0; q < h_items.size(); ++q)
@@ -683,15 +1033,9 @@ struct ImBatch {
             return -1;
         const int ni = (int)sel.size();
         (void)hipEventRecord(c->t0, s);
-        if (mode == IM_SCORE)
-            im_pass_kernel<IM_SCORE><<<ni, 64, 0, s>>>(d_items, ni, d_cand, d_rec, d_cs, kmax, R.cols64,
-                                                        R.w64, R.n, R.v, metric, d_part, IM_STATS, d_bad, nullptr);
-        else if (mode == IM_RSS)
-            im_pass_kernel<IM_RSS><<<ni, 64, 0, s>>>(d_items, ni, d_cand, d_rec, d_cs, kmax, R.cols64,
-                                                      R.w64, R.n, R.v, metric, d_part, IM_STATS, d_bad, nullptr);
-        else
-            im_pass_kernel<IM_JAC><<<ni, 64, 0, s>>>(d_items, ni, d_cand, d_rec, d_cs, kmax, R.cols64,
-                                                      R.w64, R.n, R.v, metric, d_part, IM_STATS, d_bad, nullptr);
+        if (mode == IM_SCORE) launch<IM_SCORE>(s, ni, metric, nullptr);
+        else if (mode == IM_RSS) launch<IM_RSS>(s, ni, metric, nullptr);
+        else launch<IM_JAC>(s, ni, metric, nullptr);
         (void)hipEventRecord(c->t1, s);
         if (hipGetLastError() != hipSuccess) return -1;
         if (hipMemcpyAsync(h_part.data(), d_part, sizeof(double) * (size_t)ni * IM_STATS,
@@ -802,7 +1146,6 @@ float rc_implicit_fit_batch(void *ctx, const unsigned int *prog, const unsigned 
         return -1.0f;
     }
     hipStream_t s = cx->s_fit;
-    RcRows &R = cx->sets[0];
     const int ni = (int)B.h_items.size();
     bool ok = hipMemcpyAsync(d_pre, B.pre.data(), sizeof(int) * P, hipMemcpyHostToDevice, s) == hipSuccess &&
               hipMemcpyAsync(d_i0, item0.data(), sizeof(int) * P, hipMemcpyHostToDevice, s) == hipSuccess &&
@@ -814,15 +1157,8 @@ float rc_implicit_fit_batch(void *ctx, const unsigned int *prog, const unsigned 
                                             d_sse, d_st);
     };
     auto pass = [&](int mode) {
-        if (ni <= 0) return;
-        if (mode == IM_RSS)
-            im_pass_kernel<IM_RSS><<<ni, 64, 0, s>>>(B.d_items, ni, B.d_cand, B.d_rec, B.d_cs, kmax,
-                                                      R.cols64, R.w64, R.n, R.v, 0, B.d_part,
-                                                      IM_STATS, B.d_bad, d_run);
-        else
-            im_pass_kernel<IM_JAC><<<ni, 64, 0, s>>>(B.d_items, ni, B.d_cand, B.d_rec, B.d_cs, kmax,
-                                                      R.cols64, R.w64, R.n, R.v, 0, B.d_part,
-                                                      IM_STATS, B.d_bad, d_run);
+        if (mode == IM_RSS) B.launch<IM_RSS>(s, ni, 0, d_run);
+        else B.launch<IM_JAC>(s, ni, 0, d_run);
     };
     if (ok) {
         (void)hipEventRecord(cx->t0, s);
