@@ -63,6 +63,7 @@ constexpr int IM_NVMAX = 4;
    0..7 and one in 8..15 (a 16-element vector is indexed with one register
    move per access; shorter ones are lowered to select chains). */
 typedef double im_d16 __attribute__((ext_vector_type(16)));
+typedef double im_d4 __attribute__((ext_vector_type(4)));
 
 struct ImItem {
     int cand;
@@ -568,8 +569,13 @@ __global__ __launch_bounds__(64) void im_fast_kernel(
     const int k = d.k;
     __shared__ double J[64][IM_KMAX + 2];
     const int K1 = k + 1; /* [J | r] columns */
-    const int nstats = K1 * (K1 + 1) / 2;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0}; /* owned Gram entries (JAC) */
+    /* The Gram of [J | r] on FP64 MFMA: block (0, 0) of columns 0..15 and,
+       past 16 columns, blocks (1, 0) and (1, 1). */
+    im_d4 acc00 = {0.0, 0.0, 0.0, 0.0}, acc10 = acc00, acc11 = acc00;
+    if (MODE == IM_JAC) {
+        for (int q = lane; q < 64 * (IM_KMAX + 2); q += 64) (&J[0][0])[q] = 0.0;
+        __syncthreads();
+    }
     double s0 = 0.0, s1 = 0.0;
     int flag = 0;
     const int row_end = it.row0 + it.rows;
@@ -633,20 +639,21 @@ __global__ __launch_bounds__(64) void im_fast_kernel(
                 if (!ok || !fin) flag |= 1;
             }
             __syncthreads();
-            for (int q = 0; q < 4; ++q) {
-                const int s = lane + 64 * q;
-                if (s >= nstats) break;
-                /* packed lower triangle: s = a (a + 1) / 2 + b */
-                int a = (int)((sqrt(8.0 * s + 1.0) - 1.0) * 0.5);
-                while (a * (a + 1) / 2 > s) --a;
-                while ((a + 1) * (a + 2) / 2 <= s) ++a;
-                const int b = s - a * (a + 1) / 2;
-                double tt = acc[q];
-                /* Rolled by 8: a full unroll hoists all 128 LDS reads and
-                   spills. */
-#pragma unroll 8
-                for (int r = 0; r < 64; ++r) tt += J[r][a] * J[r][b];
-                acc[q] = tt;
+            {
+                /* Operand of lane (col, sub) for rows 4 t + sub: A and B of
+                   the MFMA are the same block of J. */
+                const int col = lane & 15, sub = lane >> 4;
+                const int col1 = (16 + col < IM_KMAX + 2) ? 16 + col : IM_KMAX + 1;
+#pragma unroll 4
+                for (int t4 = 0; t4 < 16; ++t4) {
+                    const double a0 = J[4 * t4 + sub][col];
+                    acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, acc00, 0, 0, 0);
+                    if (K1 > 16) {
+                        const double a1 = J[4 * t4 + sub][col1];
+                        acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a0, acc10, 0, 0, 0);
+                        acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, acc11, 0, 0, 0);
+                    }
+                }
             }
             __syncthreads();
         }
@@ -655,9 +662,17 @@ __global__ __launch_bounds__(64) void im_fast_kernel(
     int f = flag;
     for (int off = 32; off > 0; off >>= 1) f |= __shfl_xor(f, off, 64);
     if (MODE == IM_JAC) {
-        for (int q = 0; q < 4; ++q) {
-            const int s = lane + 64 * q;
-            if (s < nstats) p[s] = acc[q];
+        /* FP64 MFMA output: lane l, register r holds D[l/16 + 4 r][l % 16]
+           of each 16x16 block; p is the packed lower triangle
+           a (a + 1) / 2 + b. */
+        const int jj = lane & 15;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int ii = (lane >> 4) + 4 * r;
+            if (ii >= jj && ii < K1) p[ii * (ii + 1) / 2 + jj] = acc00[r];
+            const int i1 = 16 + ii;
+            if (i1 < K1) p[i1 * (i1 + 1) / 2 + jj] = acc10[r];
+            if (ii >= jj && i1 < K1) p[i1 * (i1 + 1) / 2 + 16 + jj] = acc11[r];
         }
     } else {
         const double t0 = rc_wave_sum_f64(s0);
