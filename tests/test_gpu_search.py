@@ -63,3 +63,22 @@ def test_fit_pending_on_the_device_applies_its_constants_and_refits_the_rest():
     assert d.ledger[dsp.Stage.FIT].calls == len(admitted) - sum(
         1 for c in fell.values() if c in (search.FitFallbackCause.UNSUPPORTED_PLAN,
                                           search.FitFallbackCause.TOO_MANY_CONSTANTS))
+
+
+def test_device_rate_probe_feeds_the_cost_model():
+    # dispatch.rs:514-530, 642-660: the probe's four rates are finite and
+    # positive, replace the calibration's device terms, and at the
+    # reference's calibration shape the device's fit rate is the one the
+    # model then uses.
+    rates = host.probe_device_rates(rows=262144, fits=16, programs=64)
+    print(f"probe: {rates}", flush=True)
+    for v in (rates.r_eval, rates.r_fit, rates.bw):
+        assert np.isfinite(v) and v > 0
+    assert np.isfinite(rates.launch_s) and rates.launch_s >= 0
+    anchors = dsp.default_device_rates()
+    cal = dsp.with_device_rates(dsp.Calibration(
+        r_eval=anchors.r_eval, r_fit=anchors.r_fit, bw=anchors.bw, launch_s=anchors.launch_s,
+        cpu_eval=1.5e8, cpu_fit=1.0 / 6.232), rates)
+    assert (cal.r_eval, cal.r_fit, cal.bw, cal.launch_s) == (rates.r_eval, rates.r_fit, rates.bw,
+                                                             rates.launch_s)
+    assert cal.fit_rate(int(dsp.FIT_REF_ROWS), 4, 8) == pytest.approx(rates.r_fit, rel=1e-12)

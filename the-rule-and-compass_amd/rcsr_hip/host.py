@@ -475,7 +475,9 @@ def probe_device_rates(rows: int = 2456306, fits: int = 64, programs: int = 256,
                 over `rows` rows, from the batch's device time;
       r_fit     accepted fits/s at the reference shape (FIT_REF_ROWS rows,
                 FIT_REF_CONSTANTS = 4 constants, FIT_REF_SWEEPS = 8 LM
-                iterations): `fits` candidates in one batch, wall clock;
+                iterations): `fits` candidates in one batch, wall clock; a
+                probe on fewer rows is scaled to FIT_REF_ROWS by the model's
+                own inverse scaling in the rows;
       bw        host-to-device bytes/s of re-uploading the f64 fit rows;
       launch_s  the median wall time of a one-program batch over 64 rows.
 
@@ -525,7 +527,7 @@ def probe_device_rates(rows: int = 2456306, fits: int = 64, programs: int = 256,
         wall = time.perf_counter() - t0
         if ms < 0:
             raise RuntimeError("fit probe failed")
-        r_fit = fits / wall
+        r_fit = fits / wall * (rows / dispatch.FIT_REF_ROWS)
     finally:
         ev.close()
     small = GpuEvaluator(np.ascontiguousarray(x[:64].T, np.float32).reshape(-1), 64, 3,
