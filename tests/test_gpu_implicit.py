@@ -243,3 +243,35 @@ def test_adjoint_gradients_match_central_differences():
         _say(f"tree {i}: input-gradient err {ex:.2e}, constant-gradient err {ec:.2e}")
         assert ex <= 2e-4 and ec <= 2e-4, (i, ex, ec)
     ev.close()
+
+
+@pytest.mark.parametrize("nv", [1, 2, 4])
+def test_register_sweep_input_counts(nv):
+    # The register-stack sweep is compiled per input count (1-4 columns);
+    # relations over nv columns, including every operator class of the
+    # forward tangents (unary, MUL, DIV, POW with a positive base), are
+    # scored against the oracle for both metrics.
+    rng = np.random.default_rng(40 + nv)
+    n = 3000
+    X = rng.uniform(0.2, 1.6, size=(n, nv))
+    from rcsr_hip.expr import DIV, POW, SQRT, TANH
+    v = [var(j) for j in range(nv)]
+    last = v[-1]
+    trees = [
+        node(SUB, node(SQUARE, v[0]), node(MUL, const(0.7), last)),
+        node(SUB, node(DIV, node(SIN, v[0]), node(ADD, const(1.5), last)), const(0.1)),
+        node(SUB, node(POW, v[0], const(1.7)), node(MUL, const(0.3), node(TANH, last))),
+        node(ADD, node(SQRT, node(ADD, v[0], last)), node(MUL, const(-1.2), node(EXP, node(
+            MUL, const(-0.5), v[0])))),
+    ]
+    plans = [host.gpu_fit_plan(t) for t in trees]
+    assert all(p is not None for p in plans)
+    ev = _evaluator(X)
+    for metric in (0, 1):
+        out, st, ms = ev.implicit_score_batch(plans, metric)
+        assert ms >= 0
+        for i, t in enumerate(trees):
+            want = oracle.implicit_score(t, X, None, metric)
+            assert st[i] == 0, (nv, metric, i)
+            assert out[i] == pytest.approx(want, rel=1e-9), (nv, metric, i, out[i], want)
+    ev.close()

@@ -115,3 +115,16 @@ def test_device_variation_restart_and_a_second_generation(ev):
                        init_depth=4)
     _same(dev2, ref2)
     _say(f"pool vary: two generations of {n}, {len(set(ref2['hash'].tolist()))} distinct")
+
+
+@pytest.mark.parametrize("width,islands,pop,maxc,depth", [(16, 2, 5, 6, 2), (128, 1, 40, 60, 6)])
+def test_device_variation_at_the_slot_width_limits(ev, width, islands, pop, maxc, depth):
+    # A narrow pool, where most attempts fail the width filter and fall back
+    # to random programs, and the widest supported slot with deep subtrees.
+    parents, err = _parents(islands, pop, width, seed=13)
+    _say(f"pool vary: {islands} x {pop}, width {width}, max complexity {maxc}, depth {depth}")
+    for dedup in (True, False):
+        dev = _device_vary(ev, parents, err, islands, pop, 13, 3, dedup, maxc=maxc, depth=depth)
+        ref = oracle.vary(parents, islands, pop, err, 13, 3, max_complexity=maxc, nvars=2,
+                          init_depth=depth, dedup=dedup)
+        _same(dev, ref)
