@@ -2,7 +2,7 @@
  * CPU f64 oracle for the rcsr device arm — TEST INFRASTRUCTURE ONLY.
  *
  * This library is a restatement, in C++, of the reference engine's exact
- * CPU paths that judge the device arm: the tree evaluator with its domain
+ * CPU paths that check the device arm: the tree evaluator with its domain
  * guards, the flat post-order form, the adjoint sweep, the Levenberg-Marquardt
  * constant fitter, the L1/log-squashed IRLS fitter, the weighted metric
  * scorers and the implicit-relation distance/fitter.  Only the test suite,
