@@ -1235,6 +1235,10 @@ struct RcFitState {
     int n_pass[4] = {0, 0, 0, 0};
     FtState *last_state = nullptr;
     int last_nprog = 0;
+    /* A second stream with its fork / join events: the small class
+       kernels of a pass run on it beside the large ones. */
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 void rc_fit_state_free(RcCtx *c)
@@ -1244,6 +1248,9 @@ void rc_fit_state_free(RcCtx *c)
     if (c->fit->graph) (void)hipGraphDestroy(c->fit->graph);
     if (c->fit->dev) (void)hipFree(c->fit->dev);
     if (c->fit->host) (void)hipHostFree(c->fit->host);
+    if (c->fit->side) (void)hipStreamDestroy(c->fit->side);
+    if (c->fit->ev_fork) (void)hipEventDestroy(c->fit->ev_fork);
+    if (c->fit->ev_join) (void)hipEventDestroy(c->fit->ev_join);
     delete c->fit;
     c->fit = nullptr;
 }
@@ -1631,6 +1638,13 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     int *d_pbad = (int *)(D + o_pbad);
     int *d_pskip = (int *)(D + o_pskip);
 
+    if ((!f->side && hipStreamCreateWithFlags(&f->side, hipStreamNonBlocking) != hipSuccess) ||
+        (!f->ev_fork && hipEventCreateWithFlags(&f->ev_fork, hipEventDisableTiming) != hipSuccess) ||
+        (!f->ev_join && hipEventCreateWithFlags(&f->ev_join, hipEventDisableTiming) != hipSuccess)) {
+        (void)hipGetLastError();
+        return -1.0f;
+    }
+
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
     auto mark = [&](int kind, auto &&fn) {
         if (!f->profile) { fn(); return; }
@@ -1648,87 +1662,104 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
                                             d_pskip, d_outc, d_sse, d_st, all64 ? 1 : 0);
     };
     auto ctrl = [&](int op) { mark(3, [&]() { ctrl0(op); }); };
-    auto asm_launch = [&](auto kern, int q) {
+    /* The classes of a pass write disjoint items: the large ones run on
+       the fit stream, the small ones on the side stream between a fork and
+       a join, so the small classes fill the slots the large ones leave. */
+    hipStream_t &sd = f->side;
+    auto fork = [&]() {
+        (void)hipEventRecord(f->ev_fork, s);
+        (void)hipStreamWaitEvent(sd, f->ev_fork, 0);
+    };
+    auto join = [&]() {
+        (void)hipEventRecord(f->ev_join, sd);
+        (void)hipStreamWaitEvent(s, f->ev_join, 0);
+    };
+    auto asm_launch = [&](auto kern, int q, hipStream_t st) {
         const int nl = lofs[q + 1] - lofs[q];
         if (nl <= 0) return;
-        kern<<<nl, 64, 0, s>>>(d_items, d_lists + lofs[q], nl, d_cand, d_rows, d_code, d_run_a,
-                               d_cp32, d_cp64, d_pgram, tri_stride, d_ploss, d_pbad, d_pskip);
+        kern<<<nl, 64, 0, st>>>(d_items, d_lists + lofs[q], nl, d_cand, d_rows, d_code, d_run_a,
+                                d_cp32, d_cp64, d_pgram, tri_stride, d_ploss, d_pbad, d_pskip);
     };
-    auto vm_asm = [&](int kc, int r, bool polish, int q) {
-        vm_launch_asm(kc, r, polish, lofs[q + 1] - lofs[q], s, d_items, d_lists + lofs[q], d_cand,
+    auto vm_asm = [&](int kc, int r, bool polish, int q, hipStream_t st) {
+        vm_launch_asm(kc, r, polish, lofs[q + 1] - lofs[q], st, d_items, d_lists + lofs[q], d_cand,
                       d_rows, d_code, d_run_a, d_cp32, d_cp64, d_pgram, tri_stride, d_ploss,
                       d_pbad, d_pskip);
     };
-    auto asm64_launch = [&](auto kern, int q) {
+    auto asm64_launch = [&](auto kern, int q, hipStream_t st) {
         const int nl = lofs[q + 1] - lofs[q];
         if (nl <= 0) return;
-        kern<<<nl, 64, 0, s>>>(d_items, d_lists + lofs[q], nl, d_cand, d_rows, d_code, d_run_a,
-                               d_cp64, d_pgram, tri_stride, d_ploss, d_pbad, d_pskip);
+        kern<<<nl, 64, 0, st>>>(d_items, d_lists + lofs[q], nl, d_cand, d_rows, d_code, d_run_a,
+                                d_cp64, d_pgram, tri_stride, d_ploss, d_pbad, d_pskip);
     };
-    auto vm_asm64 = [&](int kc, bool polish, int q) {
-        vm_launch_asm64(kc, polish, lofs[q + 1] - lofs[q], s, d_items, d_lists + lofs[q], d_cand,
+    auto vm_asm64 = [&](int kc, bool polish, int q, hipStream_t st) {
+        vm_launch_asm64(kc, polish, lofs[q + 1] - lofs[q], st, d_items, d_lists + lofs[q], d_cand,
                         d_rows, d_code, d_run_a, d_cp64, d_pgram, tri_stride, d_ploss, d_pbad,
                         d_pskip);
     };
     auto asm_pass0 = [&](int polish) {
+        fork();
         if (all64) {
             /* Classes 0 and 1 (4 f32 rows per lane) are empty here. */
-            vm_asm64(16, polish != 0, 2);
-            vm_asm64(32, polish != 0, 3);
+            vm_asm64(16, polish != 0, 2, s);
+            vm_asm64(32, polish != 0, 3, s);
             if (polish) {
-                asm64_launch(ft_asm64_kernel<16, false, true>, 4);
-                asm64_launch(ft_asm64_kernel<32, false, true>, 5);
-                asm64_launch(ft_asm64_kernel<16, true, true>, 6);
-                asm64_launch(ft_asm64_kernel<32, true, true>, 7);
+                asm64_launch(ft_asm64_kernel<16, false, true>, 4, sd);
+                asm64_launch(ft_asm64_kernel<32, false, true>, 5, sd);
+                asm64_launch(ft_asm64_kernel<16, true, true>, 6, sd);
+                asm64_launch(ft_asm64_kernel<32, true, true>, 7, sd);
             } else {
-                asm64_launch(ft_asm64_kernel<16, false, false>, 4);
-                asm64_launch(ft_asm64_kernel<32, false, false>, 5);
-                asm64_launch(ft_asm64_kernel<16, true, false>, 6);
-                asm64_launch(ft_asm64_kernel<32, true, false>, 7);
+                asm64_launch(ft_asm64_kernel<16, false, false>, 4, sd);
+                asm64_launch(ft_asm64_kernel<32, false, false>, 5, sd);
+                asm64_launch(ft_asm64_kernel<16, true, false>, 6, sd);
+                asm64_launch(ft_asm64_kernel<32, true, false>, 7, sd);
             }
         } else if (polish) {
-            vm_asm(16, 4, true, 0);
-            vm_asm(32, 4, true, 1);
-            vm_asm(16, 2, true, 2);
-            vm_asm(32, 2, true, 3);
-            asm_launch(ft_asm_kernel<16, false, true>, 4);
-            asm_launch(ft_asm_kernel<32, false, true>, 5);
-            asm_launch(ft_asm_kernel<16, true, true>, 6);
-            asm_launch(ft_asm_kernel<32, true, true>, 7);
+            vm_asm(16, 4, true, 0, s);
+            vm_asm(32, 4, true, 1, s);
+            vm_asm(16, 2, true, 2, sd);
+            vm_asm(32, 2, true, 3, sd);
+            asm_launch(ft_asm_kernel<16, false, true>, 4, sd);
+            asm_launch(ft_asm_kernel<32, false, true>, 5, sd);
+            asm_launch(ft_asm_kernel<16, true, true>, 6, sd);
+            asm_launch(ft_asm_kernel<32, true, true>, 7, sd);
         } else {
-            vm_asm(16, 4, false, 0);
-            vm_asm(32, 4, false, 1);
-            vm_asm(16, 2, false, 2);
-            vm_asm(32, 2, false, 3);
-            asm_launch(ft_asm_kernel<16, false, false>, 4);
-            asm_launch(ft_asm_kernel<32, false, false>, 5);
-            asm_launch(ft_asm_kernel<16, true, false>, 6);
-            asm_launch(ft_asm_kernel<32, true, false>, 7);
+            vm_asm(16, 4, false, 0, s);
+            vm_asm(32, 4, false, 1, s);
+            vm_asm(16, 2, false, 2, sd);
+            vm_asm(32, 2, false, 3, sd);
+            asm_launch(ft_asm_kernel<16, false, false>, 4, sd);
+            asm_launch(ft_asm_kernel<32, false, false>, 5, sd);
+            asm_launch(ft_asm_kernel<16, true, false>, 6, sd);
+            asm_launch(ft_asm_kernel<32, true, false>, 7, sd);
         }
+        join();
     };
     auto asm_pass = [&](int polish) { mark(0, [&]() { asm_pass0(polish); }); };
     auto trial_pass0 = [&](bool exact_) {
         const bool exact = exact_ || all64;
+        fork();
         for (int q = 0; q < NT; ++q) {
             const int nl = tofs[q + 1] - tofs[q];
             if (nl <= 0) continue;
             const int *L = d_lists + tofs[q];
+            const hipStream_t st = q == 0 ? s : sd;
             if (!exact) {
                 if (q == 0)
-                    vm_launch_trial(false, nl, s, d_items, L, d_cand, d_rows, d_code, d_run_t, d_cp32, d_cp64, d_ploss);
+                    vm_launch_trial(false, nl, st, d_items, L, d_cand, d_rows, d_code, d_run_t, d_cp32, d_cp64, d_ploss);
                 else if (q == 1)
-                    ft_trial_kernel<float, false><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
+                    ft_trial_kernel<float, false><<<nl, 64, 0, st>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
                 else
-                    ft_trial_kernel<float, true><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
+                    ft_trial_kernel<float, true><<<nl, 64, 0, st>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
             } else {
                 if (q == 0)
-                    vm_launch_trial(true, nl, s, d_items, L, d_cand, d_rows, d_code, d_run_t, d_cp32, d_cp64, d_ploss);
+                    vm_launch_trial(true, nl, st, d_items, L, d_cand, d_rows, d_code, d_run_t, d_cp32, d_cp64, d_ploss);
                 else if (q == 1)
-                    ft_trial_kernel<double, false><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
+                    ft_trial_kernel<double, false><<<nl, 64, 0, st>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
                 else
-                    ft_trial_kernel<double, true><<<nl, 64, 0, s>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
+                    ft_trial_kernel<double, true><<<nl, 64, 0, st>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
             }
         }
+        join();
     };
 
     auto trial_pass = [&](bool exact) { mark(exact ? 2 : 1, [&]() { trial_pass0(exact); }); };
