@@ -74,6 +74,7 @@ struct FtState {
     int status, sstart, srank, improved, skipped;
     int halvings, polish_it, pre_status, acc_bits; /* acc_bits: LM decision q accepted (bit q) */
     int n_asm, n_trial, n_exact, n_decided; /* passes consumed, LM decisions (telemetry) */
+    int pruned; /* this LM trial was rejected on its first half (ft_trial_prune) */
 };
 
 /* Wave-uniform storage for the interpreter: an LLVM vector (registers,
@@ -921,7 +922,8 @@ __device__ __forceinline__ int ctrl_lane0(int op, int cand, FtState *s, const Ft
         }
         case CT_AFTER_TRIAL: {
             if (!ran_trial || s->stage != ST_NEED_TRIAL) break;
-            const double tsse = sh_loss;
+            const double tsse = s->pruned ? __builtin_inf() : sh_loss;
+            s->pruned = 0;
             if (!(tsse < s->best) && s->halvings > 0) {
                 /* True-metric Armijo halving for the L1 arm (eval.cu:2496-2509). */
                 s->halvings -= 1;
@@ -1088,6 +1090,34 @@ __global__ void ft_state_init(FtState *S, int ncand, const int *pre_status, int 
     if (c >= ncand) return;
     S[c].pre_status = pre_status[c];
     S[c].iters = iters;
+    S[c].pruned = 0;
+}
+
+/* Between the two halves of an LM trial pass.  A candidate's trial loss is
+   the sum of its items' losses in item order (the control kernel's
+   reduction) and every term is non-negative, so once the sum of its first
+   (nitems + 1) / 2 items, taken in the same order, reaches the best loss,
+   the whole sum does too and the trial is rejected whatever the rest adds:
+   the second half of its items is skipped (run_b = 0) and the control
+   kernel reads the trial as rejected.  The accept / reject sequence and
+   every accepted loss are those of the full pass. */
+__global__ void ft_trial_prune(int nprog, FtState *__restrict__ S, const FtCand *__restrict__ cands,
+                               const double *__restrict__ ploss, const int *__restrict__ run_t,
+                               int *__restrict__ run_b)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nprog) return;
+    FtState *st = S + c;
+    int pr = 0;
+    if (run_t[c] && st->stage == ST_NEED_TRIAL && isfinite(st->best)) {
+        const FtCand d = cands[c];
+        const int h = (d.nitems + 1) / 2;
+        double acc = 0.0;
+        for (int t = 0; t < h; ++t) acc += ploss[d.item0 + t];
+        pr = acc >= st->best ? 1 : 0;
+    }
+    st->pruned = pr;
+    run_b[c] = (run_t[c] && !pr) ? 1 : 0;
 }
 
 } /* namespace */
@@ -1273,7 +1303,8 @@ static size_t take(size_t &cur, size_t bytes)
 struct FtRegion {
     size_t o_code, o_cand, o_rows, o_items, o_lists, o_pre, o_c0, in_end;
     size_t o_outc, o_sse, o_st, io_end;
-    size_t o_state, o_run_a, o_run_t, o_cp32, o_cp64, o_pgram, o_ploss, o_pbad, o_pskip, dev_end;
+    size_t o_state, o_run_a, o_run_t, o_run_b, o_cp32, o_cp64, o_pgram, o_ploss, o_pbad, o_pskip,
+        dev_end;
 };
 static const int FT_REGION_FIELDS = (int)(sizeof(FtRegion) / sizeof(size_t));
 
@@ -1298,6 +1329,7 @@ static FtRegion ft_region(size_t code_words, size_t ncands, size_t nrowsets, siz
     g.o_state = take(cur, np * sizeof(FtState));
     g.o_run_a = take(cur, np * 4);
     g.o_run_t = take(cur, np * 4);
+    g.o_run_b = take(cur, np * 4);
     g.o_cp32 = take(cur, np * FT_KMAX * 4);
     g.o_cp64 = take(cur, np * FT_KMAX * 8);
     g.o_pgram = take(cur, ni * (size_t)tri_stride * 8);
@@ -1493,6 +1525,9 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     enum { NA = 8, NT = 3 };
     std::vector<int> list[NA];
     std::vector<int> tlist[NT];
+    /* The trial lists split per candidate into the first (nitems + 1) / 2
+       items and the rest, for the LM trial passes' early rejection. */
+    std::vector<int> tlist_a[NT], tlist_b[NT];
     int maxK = 2;
     const int tile_rows = c->tune_fit_tile > 0 ? c->tune_fit_tile : TILE_ROWS;
     for (int i = 0; i < nprog; ++i) {
@@ -1542,7 +1577,8 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         const int kc = d.k + 2 <= 16 ? 0 : 1;
         const int cls = d.vm ? (vm4 ? 0 : 2) + kc : (big ? 6 : 4) + kc;
         maxK = std::max(maxK, d.k + 2);
-        for (int r0 = 0; r0 < n; r0 += tile_rows) {
+        const int ntiles = (n + tile_rows - 1) / tile_rows;
+        for (int r0 = 0, tix = 0; r0 < n; r0 += tile_rows, ++tix) {
             FtItem it;
             it.cand = i;
             it.row0 = r0;
@@ -1550,6 +1586,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
             it.pad = 0;
             list[cls].push_back((int)items.size());
             tlist[tcls].push_back((int)items.size());
+            (tix < (ntiles + 1) / 2 ? tlist_a : tlist_b)[tcls].push_back((int)items.size());
             items.push_back(it);
         }
         d.nitems = (int)items.size() - d.item0;
@@ -1574,7 +1611,11 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         for (size_t j = 0; j < l.size(); ++j) l[j] = keyed[j].second;
     };
     for (int q = 0; q < NA; ++q) by_work(list[q]);
-    for (int q = 0; q < NT; ++q) by_work(tlist[q]);
+    for (int q = 0; q < NT; ++q) {
+        by_work(tlist[q]);
+        by_work(tlist_a[q]);
+        by_work(tlist_b[q]);
+    }
     const std::vector<FtRows> rowsd = rc_fit_rows(c);
     std::vector<int> flat_lists;
     int lofs[NA + 1], tofs[NT + 1];
@@ -1588,6 +1629,17 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         flat_lists.insert(flat_lists.end(), tlist[q].begin(), tlist[q].end());
     }
     tofs[NT] = (int)flat_lists.size();
+    int tofs_a[NT + 1], tofs_b[NT + 1];
+    for (int q = 0; q < NT; ++q) {
+        tofs_a[q] = (int)flat_lists.size();
+        flat_lists.insert(flat_lists.end(), tlist_a[q].begin(), tlist_a[q].end());
+    }
+    tofs_a[NT] = (int)flat_lists.size();
+    for (int q = 0; q < NT; ++q) {
+        tofs_b[q] = (int)flat_lists.size();
+        flat_lists.insert(flat_lists.end(), tlist_b[q].begin(), tlist_b[q].end());
+    }
+    tofs_b[NT] = (int)flat_lists.size();
     if (flat_lists.empty()) flat_lists.push_back(0);
 
     /* Region layout: inputs [code | cands | rows | items | lists | pre | c0]
@@ -1631,6 +1683,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     FtState *d_state = (FtState *)(D + o_state);
     int *d_run_a = (int *)(D + o_run_a);
     int *d_run_t = (int *)(D + o_run_t);
+    int *d_run_b = (int *)(D + rg.o_run_b);
     float *d_cp32 = (float *)(D + o_cp32);
     double *d_cp64 = (double *)(D + o_cp64);
     double *d_pgram = (double *)(D + o_pgram);
@@ -1735,38 +1788,54 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         join();
     };
     auto asm_pass = [&](int polish) { mark(0, [&]() { asm_pass0(polish); }); };
-    auto trial_pass0 = [&](bool exact_) {
+    /* One trial pass over the item lists at offsets of (full, first
+       halves or second halves) with the run flags run. */
+    auto trial_lists = [&](bool exact_, const int *of, const int *run) {
         const bool exact = exact_ || all64;
         fork();
         for (int q = 0; q < NT; ++q) {
-            const int nl = tofs[q + 1] - tofs[q];
+            const int nl = of[q + 1] - of[q];
             if (nl <= 0) continue;
-            const int *L = d_lists + tofs[q];
+            const int *L = d_lists + of[q];
             const hipStream_t st = q == 0 ? s : sd;
             if (!exact) {
                 if (q == 0)
-                    vm_launch_trial(false, nl, st, d_items, L, d_cand, d_rows, d_code, d_run_t, d_cp32, d_cp64, d_ploss);
+                    vm_launch_trial(false, nl, st, d_items, L, d_cand, d_rows, d_code, run, d_cp32, d_cp64, d_ploss);
                 else if (q == 1)
-                    ft_trial_kernel<float, false><<<nl, 64, 0, st>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
+                    ft_trial_kernel<float, false><<<nl, 64, 0, st>>>(d_items, L, nl, d_cand, d_rows, d_code, run, d_cp32, d_ploss);
                 else
-                    ft_trial_kernel<float, true><<<nl, 64, 0, st>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp32, d_ploss);
+                    ft_trial_kernel<float, true><<<nl, 64, 0, st>>>(d_items, L, nl, d_cand, d_rows, d_code, run, d_cp32, d_ploss);
             } else {
                 if (q == 0)
-                    vm_launch_trial(true, nl, st, d_items, L, d_cand, d_rows, d_code, d_run_t, d_cp32, d_cp64, d_ploss);
+                    vm_launch_trial(true, nl, st, d_items, L, d_cand, d_rows, d_code, run, d_cp32, d_cp64, d_ploss);
                 else if (q == 1)
-                    ft_trial_kernel<double, false><<<nl, 64, 0, st>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
+                    ft_trial_kernel<double, false><<<nl, 64, 0, st>>>(d_items, L, nl, d_cand, d_rows, d_code, run, d_cp64, d_ploss);
                 else
-                    ft_trial_kernel<double, true><<<nl, 64, 0, st>>>(d_items, L, nl, d_cand, d_rows, d_code, d_run_t, d_cp64, d_ploss);
+                    ft_trial_kernel<double, true><<<nl, 64, 0, st>>>(d_items, L, nl, d_cand, d_rows, d_code, run, d_cp64, d_ploss);
             }
         }
         join();
     };
 
+    auto trial_pass0 = [&](bool exact) { trial_lists(exact, tofs, d_run_t); };
+    /* An LM trial pass: the first halves, the prune, the second halves of
+       the candidates not yet rejected. */
+    auto trial_lm0 = [&]() {
+        trial_lists(false, tofs_a, d_run_t);
+        ft_trial_prune<<<(nprog + 63) / 64, 64, 0, s>>>(nprog, d_state, d_cand, d_ploss, d_run_t,
+                                                         d_run_b);
+        trial_lists(false, tofs_b, d_run_b);
+    };
     auto trial_pass = [&](bool exact) { mark(exact ? 2 : 1, [&]() { trial_pass0(exact); }); };
+    auto trial_lm = [&]() { mark(1, [&]() { trial_lm0(); }); };
 
     /* The launch sequence: recorded as a hipGraph per batch shape. */
-    const std::vector<uint64_t> key = ft_graph_key(nprog, nitems, iters, any_irls, kmax, tri_stride,
-                                                   D, rg, lofs, NA + 1, tofs, NT + 1, f->mode);
+    std::vector<uint64_t> key = ft_graph_key(nprog, nitems, iters, any_irls, kmax, tri_stride, D,
+                                             rg, lofs, NA + 1, tofs, NT + 1, f->mode);
+    for (int q = 0; q <= NT; ++q) {
+        key.push_back((uint64_t)tofs_a[q]);
+        key.push_back((uint64_t)tofs_b[q]);
+    }
     const char *nograph = getenv("RCSR_FIT_NO_GRAPH");
     const bool use_graph = !(nograph && nograph[0] == '1') && !f->profile;
     auto record = [&]() {
@@ -1775,11 +1844,11 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
         for (int slot = 0; slot < iters; ++slot) {
             asm_pass(0);
             ctrl(CT_AFTER_ASM);
-            trial_pass(false);
+            trial_lm();
             ctrl(CT_AFTER_TRIAL);
             if (any_irls) {
                 for (int h = 0; h < 2; ++h) {
-                    trial_pass(false);
+                    trial_lm();
                     ctrl(CT_AFTER_TRIAL);
                 }
             }
