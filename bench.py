@@ -692,7 +692,7 @@ def c4_part(host, workloads):
     fixed = fit_ms()
     ts = time.perf_counter()
     res = autotune.tune_fit(ev, plans, wl.iters, rows=rows, rowset=rowset, metrics=met,
-                            sample=512, reps=1)
+                            sample=512, reps=1, confirm=True)
     tune_s = time.perf_counter() - ts
     tuned = fit_ms()
     ev.close()

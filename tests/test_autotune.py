@@ -115,3 +115,37 @@ def test_tune_fit_uses_the_cache_and_an_even_sample(tmp_path):
     res2 = at.tune_fit(ev, plans, 30, 0, rows=[262144] * 1000, sample=100, cache=cache)
     assert len(runs) == n and math.isnan(res2.best_ms)  # applied from the cache, no timing
     assert ev.t[TUNE_FIT_TILE_ROWS] == res.best[TUNE_FIT_TILE_ROWS]
+
+
+def test_tune_fit_confirms_a_sample_winner_on_the_whole_batch():
+    # On the 100-candidate sample 4096-row items win; on the whole batch
+    # they lose to the built-in shape, which confirmation then keeps.  When
+    # the whole batch agrees with the sample, the winner stays.
+    for full_prefers_builtin in (True, False):
+        ev = FakeEv(None)
+        size = {"n": 0}
+
+        def cost(t, size=size, flip=full_prefers_builtin):
+            tile = t[TUNE_FIT_TILE_ROWS] or 8192
+            rows = t[TUNE_FIT_ROWS_PER_LANE] or 4
+            base = 10.0 if size["n"] < 1000 else 40.0
+            if rows == 2:
+                return base * 1.5
+            if tile == 4096:
+                return base * (0.9 if (size["n"] < 1000 or not flip) else 1.1)
+            return base
+        ev.cost = cost
+
+        def fit_batch(plans, iters, metric):
+            size["n"] = len(plans)
+            return [], ev.run()
+        ev.fit_batch = fit_batch
+        plans = [Plan(k % 20 + 1) for k in range(1000)]
+        res = at.tune_fit(ev, plans, 30, 0, rows=[262144] * 1000, sample=100, reps=1,
+                          confirm=True, space=[{TUNE_FIT_TILE_ROWS: t, TUNE_FIT_ROWS_PER_LANE: r}
+                                               for t in (4096, 8192) for r in (4, 2)])
+        assert res is not None
+        want = 8192 if full_prefers_builtin else 4096
+        assert res.best[TUNE_FIT_TILE_ROWS] == want
+        assert ev.t[TUNE_FIT_TILE_ROWS] == want
+        assert res.builtin_ms == 40.0

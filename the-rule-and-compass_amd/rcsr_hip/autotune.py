@@ -38,6 +38,7 @@ FIT_TILES = (2048, 4096, 8192, 16384, 32768, 65536)
 FIT_ROWS = (4, 2)
 EVAL_TILES = (1024, 2048, 4096, 8192, 16384)
 BUILTIN = {TUNE_FIT_TILE_ROWS: 8192, TUNE_FIT_ROWS_PER_LANE: 4, TUNE_EVAL_TILE_ROWS: 4096}
+BUILTIN_FIT = {TUNE_FIT_TILE_ROWS: 8192, TUNE_FIT_ROWS_PER_LANE: 4}
 
 Setting = Dict[int, int]
 
@@ -172,13 +173,16 @@ def _apply(ev, setting: Setting) -> None:
 
 def tune_fit(ev, plans, iters: int, metric: int = 0, rows: Optional[Sequence[int]] = None,
              sample: int = 256, reps: int = 2, cache: Optional[TuningCache] = None,
-             rowset=None, metrics=None, space: Optional[Sequence[Setting]] = None
-             ) -> Optional[TuneResult]:
+             rowset=None, metrics=None, space: Optional[Sequence[Setting]] = None,
+             confirm: bool = False, margin: float = 0.02) -> Optional[TuneResult]:
     """Tune the fit shapes for a batch: `rows` are the candidates' row counts
     (one per candidate); a heterogeneous batch passes its rowset and metrics
     (rc_fit_batch_hetero).  The timing runs on an evenly spread sample of at
-    most `sample` candidates.  With a cache, a known signature is applied
-    without timing (best_ms is then NaN)."""
+    most `sample` candidates.  With `confirm`, a sample winner other than the
+    built-in shape is timed against the built-in shape on the whole batch and
+    kept only if it is faster there by more than `margin` (a sample can rank
+    shapes differently from the batch it stands for).  With a cache, a known
+    signature is applied without timing (best_ms is then NaN)."""
     n = len(plans)
     rows = list(rows) if rows is not None else [0] * n
     step = max(1, n // max(sample, 1))
@@ -201,7 +205,20 @@ def tune_fit(ev, plans, iters: int, metric: int = 0, rows: Optional[Sequence[int
         def run():
             return ev.fit_batch(sub, iters, metric)[1]
     res = tune(ev, run, space if space is not None else fit_space(max(rows) if rows else 0),
-               reps=reps)
+               reps=reps, margin=margin)
+    if res is not None and confirm and not _is_builtin(res.best):
+        if rowset is not None:
+            rs_all = [int(r) for r in rowset]
+            ms_all = [int(m) for m in metrics] if metrics is not None else [metric] * n
+
+            def run_all():
+                return ev.fit_batch_hetero(plans, rs_all, ms_all, iters)[1]
+        else:
+            def run_all():
+                return ev.fit_batch(plans, iters, metric)[1]
+        full = tune(ev, run_all, [dict(BUILTIN_FIT), dict(res.best)], reps=reps, margin=margin)
+        if full is not None:
+            res = TuneResult(full.best, full.best_ms, full.builtin_ms, res.table)
     if res is not None and cache is not None:
         cache.put(sig, res.best)
     return res
