@@ -211,6 +211,8 @@ def main():
             "traffic": committed_traffic(),
             "traffic_source": TRAFFIC_SUMMARY if committed_traffic() is not None else None,
             "asm_ms_per_fit": rep.asm_ms,
+            "asm_timing": "HIP events on the fit stream around each assembly pass, whose class "
+                          "kernels run on two streams (the union of their intervals)",
             "asm_launches": rep.asm_launches,
             "trial_ms_per_fit": rep.trial_ms,
             "exact_ms_per_fit": rep.exact_ms,
