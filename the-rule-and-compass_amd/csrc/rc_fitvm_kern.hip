@@ -64,12 +64,31 @@ template <> struct VmAsmShape<4> {
     typedef vm_u64x2 FA;
 };
 
+/* Gram entries of the KC = 32 column pairing.  Product P pairs columns
+   0..15 (rows) with columns 6..21 (columns); product Q pairs the columns
+   c(i) = i < 8 ? i : 8 + i with themselves.  Triangle entry (a, b), a >= b,
+   comes from Q when both columns are below 8 or both at least 16 and from
+   P (row b, column a - 6) otherwise, so every entry of a K <= 22 triangle
+   is stored once, and each is the same dot product over the same rows as
+   in a blockwise 32 x 32 Gram.  (i, j) is the caller's position in both
+   16 x 16 products. */
+__device__ __forceinline__ void vm_pair32_store(double *pg, int i, int j, int K, double p,
+                                                double q)
+{
+    const int a = 6 + j;
+    if (j >= 2 && i <= a && a < K) pg[a * (a + 1) / 2 + i] = p;
+    const int ci = i < 8 ? i : 8 + i, cj = j < 8 ? j : 8 + j;
+    if (i >= j && ci < K && (ci < 8 || cj >= 16)) pg[ci * (ci + 1) / 2 + cj] = q;
+}
+
 /* Assembly on the fit VM: the same per-row contract as ft_asm_kernel (the
    augmented row [s*J | s*r_hi | s*r_lo] of every row into the Gram block,
    the loss sum, bad-row and skipped-row flags), 64 R rows per wave step.
    The streams hold the inputs in U slots 0..v-1.  For KC = 32 only the
-   K <= 22 live columns are stored; the MFMA reads of the padding columns
-   are clamped to the last stored one and land only in discarded entries.
+   K <= 22 live columns are stored, and the Gram takes two MFMAs per four
+   rows (the column pairing of vm_pair32_store) where three 16 x 16 blocks
+   of a 32 x 32 Gram would take three; operand reads beyond the stored
+   columns are clamped to the last one and land only in discarded entries.
    The LDS block allows at most two waves per SIMD.  The polish variant,
    whose f64 residual sweep needs more registers than two waves hold, is
    still held to two (it spills a few hundred bytes per lane, and measured
@@ -90,6 +109,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(POLISH ? 2 :
     typedef VmPack<V> P;
     constexpr int XS = VM_XS(R);
     constexpr int KCA = (KC == 32) ? FT_KMAX + 2 : KC; /* stored columns */
+    static_assert(KC != 32 || KCA == 22, "the KC = 32 column pairing covers 22 columns");
     constexpr int BLOCK = 64 * R;
     const int li = blockIdx.x;
     if (li >= nlist) return;
@@ -111,8 +131,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(POLISH ? 2 :
     const uint2 *fc = (const uint2 *)(code + d.fwd_off);
     const uint2 *rc = (const uint2 *)(code + d.rev_off);
     char *xl = (char *)(X + lane);
-    floatx4 acc00 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
-    double g00[4] = {0, 0, 0, 0}, g10[4] = {0, 0, 0, 0}, g11[4] = {0, 0, 0, 0};
+    /* KC = 16: acc00 is the Gram block.  KC = 32: acc00 and acc10 are the
+       products P and Q of vm_pair32_store. */
+    floatx4 acc00 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0};
+    double g00[4] = {0, 0, 0, 0}, g10[4] = {0, 0, 0, 0};
     double lsum = 0.0;
     int bad = 0, skip = 0;
     FU U0 = (FU)(0), U1 = U0;
@@ -226,43 +248,46 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(POLISH ? 2 :
         __syncthreads();
         const int col = lane & 15;
         const int sub = lane >> 4;
-        const int col1 = (16 + col < KCA) ? 16 + col : KCA - 1;
-        /* Double-buffered operands: the LDS reads of the next GQ MFMAs are
+        const int colc = col < 8 ? col : (8 + col < KCA ? 8 + col : KCA - 1);
+        /* Double-buffered operands: the LDS reads of the next GQ groups are
            issued before the current GQ run (the scheduling fences keep them
            there), so no MFMA waits on a fresh LDS read.  The MFMAs see the
            same operands in the same order. */
-        constexpr int NQ = BLOCK / 4, GQ = 8;
+        constexpr int NQ = BLOCK / 4, GQ = KC == 32 ? 4 : 8;
         static_assert(NQ % (2 * GQ) == 0, "operand batch pairs tile the block");
         const float *x0 = X + col * XS + sub;
-        const float *x1 = X + col1 * XS + sub;
-        float p0[GQ], p1[GQ], q0[GQ], q1[GQ];
-        auto load = [&](float *d0, float *d1, int g) {
+        const float *x1 = X + (6 + col) * XS + sub;
+        const float *x2 = X + colc * XS + sub;
+        float p0[GQ], p1[GQ], p2[GQ], q0[GQ], q1[GQ], q2[GQ];
+        auto load = [&](float *d0, float *d1, float *d2, int g) {
 #pragma unroll
             for (int i = 0; i < GQ; ++i) {
                 d0[i] = x0[4 * (g + i)];
                 d1[i] = KC == 32 ? x1[4 * (g + i)] : 0.0f;
+                d2[i] = KC == 32 ? x2[4 * (g + i)] : 0.0f;
             }
         };
-        auto gram = [&](const float *b0, const float *b1) {
+        auto gram = [&](const float *b0, const float *b1, const float *b2) {
 #pragma unroll
             for (int i = 0; i < GQ; ++i) {
-                acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0[i], b0[i], acc00, 0, 0, 0);
                 if (KC == 32) {
-                    acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1[i], b0[i], acc10, 0, 0, 0);
-                    acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1[i], b1[i], acc11, 0, 0, 0);
+                    acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0[i], b1[i], acc00, 0, 0, 0);
+                    acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(b2[i], b2[i], acc10, 0, 0, 0);
+                } else {
+                    acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0[i], b0[i], acc00, 0, 0, 0);
                 }
             }
         };
-        load(p0, p1, 0);
+        load(p0, p1, p2, 0);
 #pragma unroll
         for (int g = 0; g < NQ; g += 2 * GQ) {
-            load(q0, q1, g + GQ);
+            load(q0, q1, q2, g + GQ);
             __builtin_amdgcn_sched_barrier(0);
-            gram(p0, p1);
+            gram(p0, p1, p2);
             __builtin_amdgcn_sched_barrier(0);
-            if (g + 2 * GQ < NQ) load(p0, p1, g + 2 * GQ);
+            if (g + 2 * GQ < NQ) load(p0, p1, p2, g + 2 * GQ);
             __builtin_amdgcn_sched_barrier(0);
-            gram(q0, q1);
+            gram(q0, q1, q2);
             __builtin_amdgcn_sched_barrier(0);
         }
         __syncthreads();
@@ -274,9 +299,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(POLISH ? 2 :
                 acc00[r] = 0.0f;
                 if (KC == 32) {
                     g10[r] += (double)acc10[r];
-                    g11[r] += (double)acc11[r];
                     acc10[r] = 0.0f;
-                    acc11[r] = 0.0f;
                 }
             }
         }
@@ -284,10 +307,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(POLISH ? 2 :
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         g00[r] += (double)acc00[r];
-        if (KC == 32) {
-            g10[r] += (double)acc10[r];
-            g11[r] += (double)acc11[r];
-        }
+        if (KC == 32) g10[r] += (double)acc10[r];
     }
     const int K = k + 2;
     double *pg = pgram + (size_t)item * (size_t)tri_stride;
@@ -295,12 +315,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(POLISH ? 2 :
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int i = 4 * (lane >> 4) + r;
-        if (i >= j && i < K) pg[i * (i + 1) / 2 + j] = g00[r];
-        if (KC == 32) {
-            const int i1 = 16 + i;
-            if (i1 < K) pg[i1 * (i1 + 1) / 2 + j] = g10[r];
-            if (i >= j && i1 < K) pg[i1 * (i1 + 1) / 2 + 16 + j] = g11[r];
-        }
+        if (KC == 16 && i >= j && i < K) pg[i * (i + 1) / 2 + j] = g00[r];
+        if (KC == 32) vm_pair32_store(pg, i, j, K, g00[r], g10[r]);
     }
     const double ls = rc_wave_sum_f64(lsum);
     int sk = skip;
@@ -340,6 +356,7 @@ __global__ __launch_bounds__(64) void vm_asm64_kernel(
     constexpr int R = 2;
     constexpr int XS = VM64_XS(R);
     constexpr int KCA = (KC == 32) ? FT_KMAX + 2 : KC;
+    static_assert(KC != 32 || KCA == 22, "the KC = 32 column pairing covers 22 columns");
     constexpr int BLOCK = 64 * R;
     const int li = blockIdx.x;
     if (li >= nlist) return;
@@ -360,7 +377,7 @@ __global__ __launch_bounds__(64) void vm_asm64_kernel(
     const uint2 *fc = (const uint2 *)(code + d.fwd_off);
     const uint2 *rc = (const uint2 *)(code + d.rev_off);
     char *xl = (char *)(X + lane);
-    double4v acc00 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
+    double4v acc00 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0};
     double lsum = 0.0;
     int bad = 0, skip = 0;
     FU U0 = (FU)(0.0), U1 = U0;
@@ -427,17 +444,19 @@ __global__ __launch_bounds__(64) void vm_asm64_kernel(
         __syncthreads();
         const int col = lane & 15;
         const int sub = lane >> 4;
-        const int col1 = (16 + col < KCA) ? 16 + col : KCA - 1;
+        const int colc = col < 8 ? col : (8 + col < KCA ? 8 + col : KCA - 1);
         const double *x0 = X + col * XS + sub;
-        const double *x1 = X + col1 * XS + sub;
+        const double *x1 = X + (6 + col) * XS + sub;
+        const double *x2 = X + colc * XS + sub;
 #pragma unroll 8
         for (int g = 0; g < BLOCK / 4; ++g) {
             const double a0 = x0[4 * g];
-            acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, acc00, 0, 0, 0);
             if (KC == 32) {
-                const double a1 = x1[4 * g];
-                acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a0, acc10, 0, 0, 0);
-                acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, acc11, 0, 0, 0);
+                acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, x1[4 * g], acc00, 0, 0, 0);
+                const double a2 = x2[4 * g];
+                acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, a2, acc10, 0, 0, 0);
+            } else {
+                acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, acc00, 0, 0, 0);
             }
         }
         __syncthreads();
@@ -450,12 +469,8 @@ __global__ __launch_bounds__(64) void vm_asm64_kernel(
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int i = (lane >> 4) + 4 * r;
-        if (i >= j && i < K) pg[i * (i + 1) / 2 + j] = acc00[r];
-        if (KC == 32) {
-            const int i1 = 16 + i;
-            if (i1 < K) pg[i1 * (i1 + 1) / 2 + j] = acc10[r];
-            if (i >= j && i1 < K) pg[i1 * (i1 + 1) / 2 + 16 + j] = acc11[r];
-        }
+        if (KC == 16 && i >= j && i < K) pg[i * (i + 1) / 2 + j] = acc00[r];
+        if (KC == 32) vm_pair32_store(pg, i, j, K, acc00[r], acc10[r]);
     }
     const double ls = rc_wave_sum_f64(lsum);
     int sk = skip;
