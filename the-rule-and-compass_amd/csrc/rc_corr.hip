@@ -189,23 +189,16 @@ __global__ __launch_bounds__(64) void cr_ctrl_kernel(
                              s->stage == CS_START_SCORE);
     if (asm_step) {
         const int q = p + 3;
-        for (int t = lane; t < q * (q + 1) / 2; t += 64) {
-            double acc = 0.0;
-            for (int u = 0; u < d.nitems; ++u) acc += pgram[(size_t)(d.item0 + u) * CR_TRI + t];
-            G[t] = acc;
-        }
+        for (int t = lane; t < q * (q + 1) / 2; t += 64)
+            G[t] = rc_ordered_sum(pgram + (size_t)d.item0 * CR_TRI + t, CR_TRI, d.nitems);
     }
     if (lane == 0) {
         int b = 0;
-        if (asm_step)
-            for (int u = 0; u < d.nitems; ++u) b |= pbad_a[d.item0 + u];
+        if (asm_step) b |= rc_batched_or(pbad_a + d.item0, d.nitems);
         if (score_step) {
-            double mo[CR_NMOM] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-            for (int u = 0; u < d.nitems; ++u) {
-                b |= pbad_s[d.item0 + u];
-                for (int qd = 0; qd < CR_NMOM; ++qd) mo[qd] += pmom[(size_t)(d.item0 + u) * CR_NMOM + qd];
-            }
-            for (int qd = 0; qd < CR_NMOM; ++qd) sh_mom[qd] = mo[qd];
+            b |= rc_batched_or(pbad_s + d.item0, d.nitems);
+            for (int qd = 0; qd < CR_NMOM; ++qd)
+                sh_mom[qd] = rc_ordered_sum(pmom + (size_t)d.item0 * CR_NMOM + qd, CR_NMOM, d.nitems);
         }
         sh_bad = b;
     }

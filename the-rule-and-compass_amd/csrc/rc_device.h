@@ -292,4 +292,38 @@ __device__ __forceinline__ int rc_wave_or(int v)
     return __any(v) ? 1 : 0;
 }
 
+/* ---- Per-candidate reductions over work items (one lane).
+   p[0] + p[stride] + ... + p[(n - 1) stride], added left to right (the value
+   of the plain loop), with the loads issued eight at a time so that their
+   latencies overlap instead of adding up.  n >= 0 (in the last batch an
+   index past the end repeats the last element, which is loaded but not
+   added). */
+__device__ __forceinline__ double rc_ordered_sum(const double *__restrict__ p, size_t stride, int n)
+{
+    double acc = 0.0;
+    for (int t0 = 0; t0 < n; t0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[(size_t)min(t0 + u, n - 1) * stride];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (t0 + u < n) acc += v[u];
+    }
+    return acc;
+}
+
+/* p[0] | p[1] | ... | p[n - 1], n >= 0, loads issued eight at a time. */
+__device__ __forceinline__ int rc_batched_or(const int *__restrict__ p, int n)
+{
+    int acc = 0;
+    for (int t0 = 0; t0 < n; t0 += 8) {
+        int v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[min(t0 + u, n - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc |= v[u];
+    }
+    return acc;
+}
+
 #endif

@@ -75,7 +75,9 @@ struct FtState {
     int halvings, polish_it, pre_status, acc_bits; /* acc_bits: LM decision q accepted (bit q) */
     int n_asm, n_trial, n_exact, n_decided; /* passes consumed, LM decisions (telemetry) */
     int pruned; /* this LM trial was rejected on its first half (ft_trial_prune) */
+    int pad_[2];
 };
+static_assert(sizeof(FtState) % 16 == 0, "FtState is copied in 16-byte words");
 
 /* Wave-uniform storage for the interpreter: an LLVM vector (registers,
    indexed through s_set_gpr_idx) for the common sizes, a plain array for the
@@ -724,13 +726,19 @@ __global__ __launch_bounds__(64) void ft_ctrl_kernel(
     /* The candidate's state is staged in LDS for this launch: the serial
        control below (lane 0) and its solve then read and write LDS instead
        of global memory. */
-    __shared__ FtState sS;
-    constexpr int SW = (int)(sizeof(FtState) / sizeof(int));
-    static_assert(sizeof(FtState) % sizeof(int) == 0, "FtState is copied as ints");
+    __shared__ __align__(16) FtState sS;
+    constexpr int SW = (int)(sizeof(FtState) / sizeof(int4));
+    constexpr int SU = (SW + 63) / 64;
     {
-        const int *src = (const int *)(S + cand);
-        int *dst = (int *)&sS;
-        for (int i = lane; i < SW; i += 64) dst[i] = src[i];
+        const int4 *src = (const int4 *)(S + cand);
+        int4 *dst = (int4 *)&sS;
+        /* All loads first (lanes past the end repeat the last word, which
+           they also store, so no branch splits the loads from each other). */
+        int4 v[SU];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) v[u] = src[min(lane + 64 * u, SW - 1)];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) dst[min(lane + 64 * u, SW - 1)] = v[u];
     }
     __syncthreads();
     FtState *s = &sS;
@@ -747,12 +755,9 @@ __global__ __launch_bounds__(64) void ft_ctrl_kernel(
     const bool want_loss = ((op == CT_AFTER_TRIAL || op == CT_AFTER_TRIAL_P || op == CT_REBASE) &&
                             ran_trial) || want_gram;
     if (want_gram) {
-        for (int q = lane; q < tri; q += 64) {
-            double acc = 0.0;
-            for (int t = 0; t < d.nitems; ++t)
-                acc += pgram[(size_t)(d.item0 + t) * (size_t)tri_stride + q];
-            G[q] = acc;
-        }
+        for (int q = lane; q < tri; q += 64)
+            G[q] = rc_ordered_sum(pgram + (size_t)d.item0 * (size_t)tri_stride + q,
+                                  (size_t)tri_stride, d.nitems);
     }
     if (lane == 0 && want_gram) s->n_asm += 1;
     if (lane == 0 && want_loss && !want_gram) {
@@ -760,16 +765,25 @@ __global__ __launch_bounds__(64) void ft_ctrl_kernel(
         else s->n_exact += 1;
     }
     if (want_loss && lane == 0) {
-        double acc = 0.0;
         int b = 0, sk = 0;
-        for (int t = 0; t < d.nitems; ++t) {
-            acc += ploss[d.item0 + t];
-            if (want_gram) {
-                b |= pbad[d.item0 + t];
-                sk += pskip[d.item0 + t];
+        if (want_gram) {
+            for (int t0 = 0; t0 < d.nitems; t0 += 8) {
+                int vb[8], vs[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int t = d.item0 + min(t0 + u, d.nitems - 1);
+                    vb[u] = pbad[t];
+                    vs[u] = pskip[t];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (t0 + u < d.nitems) {
+                        b |= vb[u];
+                        sk += vs[u];
+                    }
             }
         }
-        sh_loss = acc;
+        sh_loss = rc_ordered_sum(ploss + d.item0, 1, d.nitems);
         sh_bad = b;
         sh_skip = sk;
     }
@@ -793,8 +807,8 @@ __global__ __launch_bounds__(64) void ft_ctrl_kernel(
     if (lane == 0) ctrl_flags(op, cand, s, run_asm, run_trial);
     __syncthreads();
     {
-        const int *src = (const int *)&sS;
-        int *dst = (int *)(S + cand);
+        const int4 *src = (const int4 *)&sS;
+        int4 *dst = (int4 *)(S + cand);
         for (int i = lane; i < SW; i += 64) dst[i] = src[i];
     }
 }
@@ -1112,9 +1126,7 @@ __global__ void ft_trial_prune(int nprog, FtState *__restrict__ S, const FtCand 
     if (run_t[c] && st->stage == ST_NEED_TRIAL && isfinite(st->best)) {
         const FtCand d = cands[c];
         const int h = (d.nitems + 1) / 2;
-        double acc = 0.0;
-        for (int t = 0; t < h; ++t) acc += ploss[d.item0 + t];
-        pr = acc >= st->best ? 1 : 0;
+        pr = rc_ordered_sum(ploss + d.item0, 1, h) >= st->best ? 1 : 0;
     }
     st->pruned = pr;
     run_b[c] = (run_t[c] && !pr) ? 1 : 0;

@@ -763,24 +763,15 @@ __global__ __launch_bounds__(64) void im_ctrl_kernel(
     const int i0 = item0[cand], ni = nitem[cand];
     if (op == IC_AFTER_JAC && !host_done && st.active && ran) {
         const int nst = (k + 1) * (k + 2) / 2;
-        for (int t = lane; t < nst; t += 64) {
-            double acc = 0.0;
-            for (int u = 0; u < ni; ++u) acc += part[(size_t)(i0 + u) * IM_STATS + t];
-            G[t] = acc;
-        }
+        for (int t = lane; t < nst; t += 64)
+            G[t] = rc_ordered_sum(part + (size_t)i0 * IM_STATS + t, IM_STATS, ni);
     }
     __syncthreads();
     if (lane == 0) {
         double *cc = cs + (size_t)cand * (size_t)kmax;
         auto flag_sum = [&](double &sum) {
-            int f = 0;
-            double acc = 0.0;
-            for (int u = 0; u < ni; ++u) {
-                f |= bad[i0 + u];
-                acc += part[(size_t)(i0 + u) * IM_STATS];
-            }
-            sum = acc;
-            return f;
+            sum = rc_ordered_sum(part + (size_t)i0 * IM_STATS, IM_STATS, ni);
+            return rc_batched_or(bad + i0, ni);
         };
         int next_run = 0;
         switch (op) {
