@@ -164,6 +164,36 @@ def test_a_mixed_p_batch_matches_single_fits_bit_for_bit():
         assert np.float64(a.sse).tobytes() == np.float64(b.sse).tobytes(), p
 
 
+def test_polish_reads_the_exact_pass_predictions_with_the_same_bits():
+    # The polish assembly of the 4-row fit VM classes reads each row's f64
+    # prediction from the buffer the exact pass at the same constants wrote;
+    # with the buffer capped to nothing it computes the prediction itself.
+    # Both routes give the same bits on a C3-shaped batch (every candidate
+    # polishes after 30 LM iterations).
+    wl = workloads.config3(n=8192, ncand=48, iters=30)
+    plans = [host.gpu_fit_plan(t) for t in wl.trees]
+    ev = _evaluator(wl.x, wl.y)
+    stored, ms1 = ev.fit_batch(plans, 30, 0, kmax=20)
+    old = os.environ.get("RCSR_FIT_P64_CAP")
+    os.environ["RCSR_FIT_P64_CAP"] = "0"
+    try:
+        computed, ms2 = ev.fit_batch(plans, 30, 0, kmax=20)
+    finally:
+        if old is None:
+            del os.environ["RCSR_FIT_P64_CAP"]
+        else:
+            os.environ["RCSR_FIT_P64_CAP"] = old
+    again, ms3 = ev.fit_batch(plans, 30, 0, kmax=20)
+    ev.close()
+    assert ms1 >= 0 and ms2 >= 0 and ms3 >= 0
+    assert sum(host.fit_status_usable(o.status) for o in stored) >= 24
+    for a, b, c in zip(stored, computed, again):
+        assert a.status == b.status == c.status
+        assert a.c.tobytes() == b.c.tobytes() == c.c.tobytes()
+        assert np.float64(a.sse).tobytes() == np.float64(b.sse).tobytes() == \
+            np.float64(c.sse).tobytes()
+
+
 def test_rank_deficient_and_non_finite_fixtures_are_labelled():
     n = 400
     x = (np.arange(n) / (n - 1) * 5.0 - 2.5)[:, None]

@@ -490,7 +490,7 @@ int rc_corr_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const int *
             it.cand = i;
             it.row0 = r0;
             it.rows = std::min(tile, r->n - r0);
-            it.pad = 0;
+            it.p64_blk = 0;
             alist[acls].push_back((int)items.size());
             slist[d.vm ? 0 : 1].push_back((int)items.size());
             items.push_back(it);

@@ -33,7 +33,9 @@
 #include "rc_fitvm_kern.h"
 #include "rc_internal.h"
 
+#include <limits.h>
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -43,6 +45,9 @@ namespace {
 
 constexpr int XS = 66;          /* LDS column stride of the augmented rows */
 constexpr int TILE_ROWS = 8192; /* built-in rows per work item (a multiple of 256) */
+/* Largest f64 prediction buffer of a fit (bytes): 8 bytes per row of every
+   candidate; 1024 candidates over 262 144 rows take 2 GiB. */
+constexpr size_t FT_P64_CAP = (size_t)32 << 30;
 /* Interpreter capacities.  Every vector update is unconditional (the op's
    arithmetic runs on scalars), so a slot above the live area is written each
    step: the small class needs depth + 1 value slots, adjoint depth + 2 slots
@@ -1316,12 +1321,13 @@ struct FtRegion {
     size_t o_code, o_cand, o_rows, o_items, o_lists, o_pre, o_c0, in_end;
     size_t o_outc, o_sse, o_st, io_end;
     size_t o_state, o_run_a, o_run_t, o_run_b, o_cp32, o_cp64, o_pgram, o_ploss, o_pbad, o_pskip,
-        dev_end;
+        o_p64, dev_end;
 };
 static const int FT_REGION_FIELDS = (int)(sizeof(FtRegion) / sizeof(size_t));
 
 static FtRegion ft_region(size_t code_words, size_t ncands, size_t nrowsets, size_t nitems,
-                          size_t nlist_words, int nprog, int kmax, int tri_stride)
+                          size_t nlist_words, int nprog, int kmax, int tri_stride,
+                          size_t p64_bytes = 0)
 {
     FtRegion g;
     size_t cur = 0;
@@ -1348,6 +1354,7 @@ static FtRegion ft_region(size_t code_words, size_t ncands, size_t nrowsets, siz
     g.o_ploss = take(cur, ni * 8);
     g.o_pbad = take(cur, ni * 4);
     g.o_pskip = take(cur, ni * 4);
+    g.o_p64 = take(cur, p64_bytes);
     g.dev_end = take(cur, 0);
     return g;
 }
@@ -1542,6 +1549,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     std::vector<int> tlist_a[NT], tlist_b[NT];
     int maxK = 2;
     const int tile_rows = c->tune_fit_tile > 0 ? c->tune_fit_tile : TILE_ROWS;
+    size_t p64_blocks = 0; /* f64 prediction buffer, in 64-row blocks */
     for (int i = 0; i < nprog; ++i) {
         FtCand &d = cands[i];
         memset(&d, 0, sizeof(d));
@@ -1595,7 +1603,8 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
             it.cand = i;
             it.row0 = r0;
             it.rows = std::min(tile_rows, n - r0);
-            it.pad = 0;
+            it.p64_blk = (int)std::min<size_t>(p64_blocks, (size_t)INT_MAX);
+            p64_blocks += (size_t)(it.rows + 63) / 64;
             list[cls].push_back((int)items.size());
             tlist[tcls].push_back((int)items.size());
             (tix < (ntiles + 1) / 2 ? tlist_a : tlist_b)[tcls].push_back((int)items.size());
@@ -1656,8 +1665,17 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
 
     /* Region layout: inputs [code | cands | rows | items | lists | pre | c0]
        then outputs [c | sse | status] then work. */
+    /* The exact passes of the polish store every row's f64 prediction for
+       the polish assembly that follows at the same constants (4-row fit VM
+       classes), while the buffer stays within FT_P64_CAP; beyond it the
+       polish assembly computes the prediction itself. */
+    size_t p64_cap = FT_P64_CAP;
+    if (const char *e = getenv("RCSR_FIT_P64_CAP")) p64_cap = (size_t)strtoull(e, nullptr, 10);
+    const bool use_p64 = !all64 && p64_blocks <= (size_t)INT_MAX &&
+                         p64_blocks * 64 * sizeof(double) <= p64_cap;
     const FtRegion rg = ft_region(code.size(), cands.size(), rowsd.size(), items.size(),
-                                  flat_lists.size(), nprog, kmax, tri_stride);
+                                  flat_lists.size(), nprog, kmax, tri_stride,
+                                  use_p64 ? p64_blocks * 64 * sizeof(double) : 0);
     const size_t o_code = rg.o_code, o_cand = rg.o_cand, o_rows = rg.o_rows, o_items = rg.o_items;
     const size_t o_lists = rg.o_lists, o_pre = rg.o_pre, o_c0 = rg.o_c0, in_end = rg.in_end;
     const size_t o_outc = rg.o_outc, o_sse = rg.o_sse, o_st = rg.o_st, io_end = rg.io_end;
@@ -1698,6 +1716,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     int *d_run_b = (int *)(D + rg.o_run_b);
     float *d_cp32 = (float *)(D + o_cp32);
     double *d_cp64 = (double *)(D + o_cp64);
+    double *d_p64 = use_p64 ? (double *)(D + rg.o_p64) : nullptr;
     double *d_pgram = (double *)(D + o_pgram);
     double *d_ploss = (double *)(D + o_ploss);
     int *d_pbad = (int *)(D + o_pbad);
@@ -1748,7 +1767,7 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
     auto vm_asm = [&](int kc, int r, bool polish, int q, hipStream_t st) {
         vm_launch_asm(kc, r, polish, lofs[q + 1] - lofs[q], st, d_items, d_lists + lofs[q], d_cand,
                       d_rows, d_code, d_run_a, d_cp32, d_cp64, d_pgram, tri_stride, d_ploss,
-                      d_pbad, d_pskip);
+                      d_pbad, d_pskip, d_p64);
     };
     auto asm64_launch = [&](auto kern, int q, hipStream_t st) {
         const int nl = lofs[q + 1] - lofs[q];
@@ -1812,14 +1831,16 @@ static float fit_run(RcCtx *c, const uint32_t *prog, const uint32_t *off, const 
             const hipStream_t st = q == 0 ? s : sd;
             if (!exact) {
                 if (q == 0)
-                    vm_launch_trial(false, nl, st, d_items, L, d_cand, d_rows, d_code, run, d_cp32, d_cp64, d_ploss);
+                    vm_launch_trial(false, nl, st, d_items, L, d_cand, d_rows, d_code, run, d_cp32, d_cp64, d_ploss,
+                                    nullptr);
                 else if (q == 1)
                     ft_trial_kernel<float, false><<<nl, 64, 0, st>>>(d_items, L, nl, d_cand, d_rows, d_code, run, d_cp32, d_ploss);
                 else
                     ft_trial_kernel<float, true><<<nl, 64, 0, st>>>(d_items, L, nl, d_cand, d_rows, d_code, run, d_cp32, d_ploss);
             } else {
                 if (q == 0)
-                    vm_launch_trial(true, nl, st, d_items, L, d_cand, d_rows, d_code, run, d_cp32, d_cp64, d_ploss);
+                    vm_launch_trial(true, nl, st, d_items, L, d_cand, d_rows, d_code, run, d_cp32, d_cp64, d_ploss,
+                                    d_p64);
                 else if (q == 1)
                     ft_trial_kernel<double, false><<<nl, 64, 0, st>>>(d_items, L, nl, d_cand, d_rows, d_code, run, d_cp64, d_ploss);
                 else

@@ -72,7 +72,7 @@ struct FtItem {
     int cand;
     int row0;
     int rows;
-    int pad;
+    int p64_blk; /* start of the item's rows in the f64 prediction buffer, in 64-row blocks */
 };
 
 typedef float fv8 __attribute__((ext_vector_type(8)));

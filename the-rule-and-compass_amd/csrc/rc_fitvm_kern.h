@@ -21,12 +21,15 @@
 #define VM64_XS(R) (64 * (R) + 4)
 
 /* Assembly over the items of one class list: kc 16 / 32 Gram columns,
-   r = 2 or 4 rows per lane, polish = the f64-residual polish (with r = 4 the
-   residual runs on the candidate's value-pass stream). */
+   r = 2 or 4 rows per lane, polish = the f64-residual polish.  With r = 4 the
+   polish reads each row's f64 prediction from p64 when it is non-null (the
+   exact pass at the same constants wrote it there) and otherwise computes it
+   on the candidate's value-pass stream; with r = 2 it computes it on the
+   tape stream. */
 void vm_launch_asm(int kc, int r, bool polish, int nl, hipStream_t s, const FtItem *items,
                    const int *list, const FtCand *cands, const FtRows *rows, const uint32_t *code,
                    const int *run, const float *cp32, const double *cp64, double *pgram,
-                   int tri_stride, double *ploss, int *pbad, int *pskip);
+                   int tri_stride, double *ploss, int *pbad, int *pskip, const double *p64);
 
 /* The plain FP64 arm's assembly on the fit VM (2 f64 rows per lane, FP64
    MFMA Gram): kc 16 / 32 Gram columns; polish selects the skip-and-count
@@ -36,9 +39,11 @@ void vm_launch_asm64(int kc, bool polish, int nl, hipStream_t s, const FtItem *i
                      const uint32_t *code, const int *run, const double *cp64, double *pgram,
                      int tri_stride, double *ploss, int *pbad, int *pskip);
 
-/* Trial (f32, 8 rows per lane) or exact (f64, 4 rows per lane) loss. */
+/* Trial (f32, 8 rows per lane) or exact (f64, 4 rows per lane) loss.  An
+   exact pass with p64 non-null also stores each live row's f64 prediction at
+   p64[64 * item.p64_blk + (row - item.row0)]. */
 void vm_launch_trial(bool exact, int nl, hipStream_t s, const FtItem *items, const int *list,
                      const FtCand *cands, const FtRows *rows, const uint32_t *code, const int *run,
-                     const float *cp32, const double *cp64, double *ploss);
+                     const float *cp32, const double *cp64, double *ploss, double *p64);
 
 #endif

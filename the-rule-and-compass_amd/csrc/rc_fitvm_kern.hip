@@ -89,18 +89,24 @@ __device__ __forceinline__ void vm_pair32_store(double *pg, int i, int j, int K,
    rows (the column pairing of vm_pair32_store) where three 16 x 16 blocks
    of a 32 x 32 Gram would take three; operand reads beyond the stored
    columns are clamped to the last one and land only in discarded entries.
-   The LDS block allows at most two waves per SIMD.  The polish variant,
-   whose f64 residual sweep needs more registers than two waves hold, is
-   still held to two (it spills a few hundred bytes per lane, and measured
-   1.9 ms faster per C3 fit than at one wave, with the same bits). */
-template <int KC, bool POLISH, int R>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(POLISH ? 2 : 1))) void vm_asm_kernel(
+   The LDS block allows at most two waves per SIMD.  PM selects the polish:
+   0 none; 1 the f64 prediction computed here (on the value-pass stream for
+   R = 4, the tape stream for R = 2), whose sweep needs more registers than
+   two waves hold, still held to two (it spills a few hundred bytes per
+   lane, and measured 1.9 ms faster per C3 fit than at one wave, with the
+   same bits); 2 (R = 4) the f64 prediction read from p64in, where the exact
+   pass at the same constants stored what PM = 1 computes. */
+template <int KC, int PM, int R>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PM != 0 ? 2 : 1))) void vm_asm_kernel(
     const FtItem *__restrict__ items, const int *__restrict__ list, int nlist,
     const FtCand *__restrict__ cands, const FtRows *__restrict__ rowsd,
     const uint32_t *__restrict__ code, const int *__restrict__ run, const float *__restrict__ cp32,
     const double *__restrict__ cp64, double *__restrict__ pgram, int tri_stride,
-    double *__restrict__ ploss, int *__restrict__ pbad, int *__restrict__ pskip)
+    double *__restrict__ ploss, int *__restrict__ pbad, int *__restrict__ pskip,
+    const double *__restrict__ p64in)
 {
+    constexpr bool POLISH = PM != 0;
+    static_assert(PM != 2 || R == 4, "stored predictions serve the 4-row classes");
     typedef VmAsmShape<R> S;
     typedef typename S::V V;
     typedef typename S::FU FU;
@@ -171,7 +177,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(POLISH ? 2 :
         double p64[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) p64[r] = (double)pred[r];
-        if (POLISH && R == 2) {
+        if (PM == 2) {
+            const double *pv = p64in + (size_t)it.p64_blk * 64 - it.row0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) p64[r] = pv[q[r]];
+        } else if (POLISH && R == 2) {
             /* f64 residual on the tape stream (16-slot file, 2 rows) */
             dv16 Ua = (dv16)(0.0), Ub = Ua, Ba = Ua, Bb = Ua;
 #pragma unroll
@@ -487,11 +497,12 @@ __global__ __launch_bounds__(64) void vm_asm64_kernel(
    slots 0..v-1, no tape), 8 f32 rows or 4 f64 rows per lane, with an 8-slot
    U file of four 64-bit-element vectors. */
 template <typename V>
-__global__ __launch_bounds__(64) void vm_trial_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void vm_trial_kernel(
     const FtItem *__restrict__ items, const int *__restrict__ list, int nlist,
     const FtCand *__restrict__ cands, const FtRows *__restrict__ rowsd,
     const uint32_t *__restrict__ code, const int *__restrict__ run,
-    const typename VmScalar<V>::T *__restrict__ cp, double *__restrict__ ploss)
+    const typename VmScalar<V>::T *__restrict__ cp, double *__restrict__ ploss,
+    double *__restrict__ p64out)
 {
     typedef typename VmScalar<V>::T T;
     typedef VmPack<V> P;
@@ -534,6 +545,12 @@ __global__ __launch_bounds__(64) void vm_trial_kernel(
             }
         const V pred = vm_forward<V, false>(fc, d.t_m, cst, U0, U1, U2, U3, U0, U1, U2, U3, Bd, Bd,
                                             Bd, Bd);
+        if (sizeof(T) == 8 && p64out) {
+            double *pv = p64out + (size_t)it.p64_blk * 64 - it.row0;
+#pragma unroll
+            for (int r = 0; r < RR; ++r)
+                if (live[r]) pv[xrow[r]] = (double)pred[r];
+        }
         /* All of the step's weights and targets are requested at once (one
            wait), and a row that does not count adds +0.0 (the sum is never
            -0), so the sum is the same as skipping it. */
@@ -562,24 +579,27 @@ __global__ __launch_bounds__(64) void vm_trial_kernel(
 void vm_launch_asm(int kc, int r, bool polish, int nl, hipStream_t s, const FtItem *items,
                    const int *list, const FtCand *cands, const FtRows *rows, const uint32_t *code,
                    const int *run, const float *cp32, const double *cp64, double *pgram,
-                   int tri_stride, double *ploss, int *pbad, int *pskip)
+                   int tri_stride, double *ploss, int *pbad, int *pskip, const double *p64)
 {
     if (nl <= 0) return;
-#define VM_ASM(KC, P, R)                                                                        \
-    vm_asm_kernel<KC, P, R><<<nl, 64, 0, s>>>(items, list, nl, cands, rows, code, run, cp32, cp64, \
-                                             pgram, tri_stride, ploss, pbad, pskip)
-    if (polish && r == 4) {
-        if (kc == 16) VM_ASM(16, true, 4);
-        else VM_ASM(32, true, 4);
+#define VM_ASM(KC, PM, R)                                                                       \
+    vm_asm_kernel<KC, PM, R><<<nl, 64, 0, s>>>(items, list, nl, cands, rows, code, run, cp32, cp64, \
+                                              pgram, tri_stride, ploss, pbad, pskip, p64)
+    if (polish && r == 4 && p64) {
+        if (kc == 16) VM_ASM(16, 2, 4);
+        else VM_ASM(32, 2, 4);
+    } else if (polish && r == 4) {
+        if (kc == 16) VM_ASM(16, 1, 4);
+        else VM_ASM(32, 1, 4);
     } else if (polish) {
-        if (kc == 16) VM_ASM(16, true, 2);
-        else VM_ASM(32, true, 2);
+        if (kc == 16) VM_ASM(16, 1, 2);
+        else VM_ASM(32, 1, 2);
     } else if (r == 4) {
-        if (kc == 16) VM_ASM(16, false, 4);
-        else VM_ASM(32, false, 4);
+        if (kc == 16) VM_ASM(16, 0, 4);
+        else VM_ASM(32, 0, 4);
     } else {
-        if (kc == 16) VM_ASM(16, false, 2);
-        else VM_ASM(32, false, 2);
+        if (kc == 16) VM_ASM(16, 0, 2);
+        else VM_ASM(32, 0, 2);
     }
 #undef VM_ASM
 }
@@ -605,11 +625,13 @@ void vm_launch_asm64(int kc, bool polish, int nl, hipStream_t s, const FtItem *i
 
 void vm_launch_trial(bool exact, int nl, hipStream_t s, const FtItem *items, const int *list,
                      const FtCand *cands, const FtRows *rows, const uint32_t *code, const int *run,
-                     const float *cp32, const double *cp64, double *ploss)
+                     const float *cp32, const double *cp64, double *ploss, double *p64)
 {
     if (nl <= 0) return;
     if (exact)
-        vm_trial_kernel<vm_d4><<<nl, 64, 0, s>>>(items, list, nl, cands, rows, code, run, cp64, ploss);
+        vm_trial_kernel<vm_d4><<<nl, 64, 0, s>>>(items, list, nl, cands, rows, code, run, cp64, ploss,
+                                                  p64);
     else
-        vm_trial_kernel<vm_f8><<<nl, 64, 0, s>>>(items, list, nl, cands, rows, code, run, cp32, ploss);
+        vm_trial_kernel<vm_f8><<<nl, 64, 0, s>>>(items, list, nl, cands, rows, code, run, cp32, ploss,
+                                                  nullptr);
 }
