@@ -49,6 +49,8 @@ __global__ __launch_bounds__(64) void cr_asm_kernel(
     constexpr int R = VM ? 2 : 1;
     constexpr int XS = VM64_XS(R);
     constexpr int KCA = (KC == 32) ? CR_ZMAX : KC;
+    constexpr int PS = KCA - 16; /* the column pairing's shift (ft_pair_store) */
+    static_assert(KC != 32 || (PS >= 0 && PS <= 8), "the column pairing covers KCA columns");
     constexpr int BLOCK = 64 * R;
     const int li = blockIdx.x;
     if (li >= nlist) return;
@@ -66,7 +68,9 @@ __global__ __launch_bounds__(64) void cr_asm_kernel(
     __syncthreads();
 
     const double *cst = cp + (size_t)cand * FT_KMAX;
-    cr_d4 acc00 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
+    /* KC = 16: acc00 is the Gram block.  KC = 32: acc00 and acc10 are the
+       products P and Q of ft_pair_store<PS>. */
+    cr_d4 acc00 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0};
     int bad = 0;
     dv16 U0 = (dv16)(0.0), U1 = U0, B0 = U0, B1 = U0;
     dv8 A0 = (dv8)(0.0), A1 = A0;
@@ -142,17 +146,19 @@ __global__ __launch_bounds__(64) void cr_asm_kernel(
         __syncthreads();
         const int col = lane & 15;
         const int sub = lane >> 4;
-        const int col1 = (16 + col < KCA) ? 16 + col : KCA - 1;
+        const int colc = col < 8 ? col : (8 + col < KCA ? 8 + col : KCA - 1);
         const double *x0 = X + col * XS + sub;
-        const double *x1 = X + col1 * XS + sub;
+        const double *x1 = X + (KC == 32 ? PS + col : col) * XS + sub;
+        const double *x2 = X + colc * XS + sub;
 #pragma unroll 8
         for (int gq = 0; gq < BLOCK / 4; ++gq) {
             const double a0 = x0[4 * gq];
-            acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, acc00, 0, 0, 0);
             if (KC == 32) {
-                const double a1 = x1[4 * gq];
-                acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a0, acc10, 0, 0, 0);
-                acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, acc11, 0, 0, 0);
+                acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, x1[4 * gq], acc00, 0, 0, 0);
+                const double a2 = x2[4 * gq];
+                acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, a2, acc10, 0, 0, 0);
+            } else {
+                acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, acc00, 0, 0, 0);
             }
         }
         __syncthreads();
@@ -164,12 +170,8 @@ __global__ __launch_bounds__(64) void cr_asm_kernel(
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int i = (lane >> 4) + 4 * r;
-        if (i >= j && i < q) pg[i * (i + 1) / 2 + j] = acc00[r];
-        if (KC == 32) {
-            const int i1 = 16 + i;
-            if (i1 < q) pg[i1 * (i1 + 1) / 2 + j] = acc10[r];
-            if (i >= j && i1 < q) pg[i1 * (i1 + 1) / 2 + 16 + j] = acc11[r];
-        }
+        if (KC == 16 && i >= j && i < q) pg[i * (i + 1) / 2 + j] = acc00[r];
+        if (KC == 32) ft_pair_store<PS>(pg, i, j, q, acc00[r], acc10[r]);
     }
     const int anybad = __any(bad) ? 1 : 0;
     if (lane == 0) pbad[item] = anybad;

@@ -57,6 +57,25 @@ struct FtCand {
     int t_m;        /* its instruction count */
 };
 
+/* Gram entries of a column pairing for K <= S + 16 <= 24 columns in two
+   16 x 16 products per four rows (where a blockwise 32 x 32 Gram takes
+   three): P pairs columns 0..15 (rows) with columns S..S+15 (columns), Q
+   pairs the columns c(i) = i < 8 ? i : 8 + i with themselves (S <= 8).
+   Triangle entry (a, b), a >= b, comes from P (row b, column a - S) when
+   a >= 8 and b < 16, else from Q (both below 8 or both at least 16), so
+   every entry is stored once and is the same dot product over the same rows
+   as in the blockwise Gram.  (i, j) is the caller's position in both
+   products. */
+template <int S>
+__device__ __forceinline__ void ft_pair_store(double *pg, int i, int j, int K, double p, double q)
+{
+    static_assert(S >= 0 && S <= 8, "Q covers the columns below 8 and from 16");
+    const int a = S + j;
+    if (a >= 8 && i <= a && a < K) pg[a * (a + 1) / 2 + i] = p;
+    const int ci = i < 8 ? i : 8 + i, cj = j < 8 ? j : 8 + j;
+    if (i >= j && ci < K && (ci < 8 || cj >= 16)) pg[ci * (ci + 1) / 2 + cj] = q;
+}
+
 /* One resident row set as the kernels see it. */
 struct FtRows {
     const float *cols;

@@ -64,29 +64,12 @@ template <> struct VmAsmShape<4> {
     typedef vm_u64x2 FA;
 };
 
-/* Gram entries of the KC = 32 column pairing.  Product P pairs columns
-   0..15 (rows) with columns 6..21 (columns); product Q pairs the columns
-   c(i) = i < 8 ? i : 8 + i with themselves.  Triangle entry (a, b), a >= b,
-   comes from Q when both columns are below 8 or both at least 16 and from
-   P (row b, column a - 6) otherwise, so every entry of a K <= 22 triangle
-   is stored once, and each is the same dot product over the same rows as
-   in a blockwise 32 x 32 Gram.  (i, j) is the caller's position in both
-   16 x 16 products. */
-__device__ __forceinline__ void vm_pair32_store(double *pg, int i, int j, int K, double p,
-                                                double q)
-{
-    const int a = 6 + j;
-    if (j >= 2 && i <= a && a < K) pg[a * (a + 1) / 2 + i] = p;
-    const int ci = i < 8 ? i : 8 + i, cj = j < 8 ? j : 8 + j;
-    if (i >= j && ci < K && (ci < 8 || cj >= 16)) pg[ci * (ci + 1) / 2 + cj] = q;
-}
-
 /* Assembly on the fit VM: the same per-row contract as ft_asm_kernel (the
    augmented row [s*J | s*r_hi | s*r_lo] of every row into the Gram block,
    the loss sum, bad-row and skipped-row flags), 64 R rows per wave step.
    The streams hold the inputs in U slots 0..v-1.  For KC = 32 only the
    K <= 22 live columns are stored, and the Gram takes two MFMAs per four
-   rows (the column pairing of vm_pair32_store) where three 16 x 16 blocks
+   rows (the column pairing of ft_pair_store) where three 16 x 16 blocks
    of a 32 x 32 Gram would take three; operand reads beyond the stored
    columns are clamped to the last one and land only in discarded entries.
    The LDS block allows at most two waves per SIMD.  PM selects the polish:
@@ -138,7 +121,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PM != 0 ? 2 
     const uint2 *rc = (const uint2 *)(code + d.rev_off);
     char *xl = (char *)(X + lane);
     /* KC = 16: acc00 is the Gram block.  KC = 32: acc00 and acc10 are the
-       products P and Q of vm_pair32_store. */
+       products P and Q of ft_pair_store<6>. */
     floatx4 acc00 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0};
     double g00[4] = {0, 0, 0, 0}, g10[4] = {0, 0, 0, 0};
     double lsum = 0.0;
@@ -326,7 +309,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PM != 0 ? 2 
     for (int r = 0; r < 4; ++r) {
         const int i = 4 * (lane >> 4) + r;
         if (KC == 16 && i >= j && i < K) pg[i * (i + 1) / 2 + j] = g00[r];
-        if (KC == 32) vm_pair32_store(pg, i, j, K, g00[r], g10[r]);
+        if (KC == 32) ft_pair_store<6>(pg, i, j, K, g00[r], g10[r]);
     }
     const double ls = rc_wave_sum_f64(lsum);
     int sk = skip;
@@ -480,7 +463,7 @@ __global__ __launch_bounds__(64) void vm_asm64_kernel(
     for (int r = 0; r < 4; ++r) {
         const int i = (lane >> 4) + 4 * r;
         if (KC == 16 && i >= j && i < K) pg[i * (i + 1) / 2 + j] = acc00[r];
-        if (KC == 32) vm_pair32_store(pg, i, j, K, acc00[r], acc10[r]);
+        if (KC == 32) ft_pair_store<6>(pg, i, j, K, acc00[r], acc10[r]);
     }
     const double ls = rc_wave_sum_f64(lsum);
     int sk = skip;
