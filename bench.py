@@ -710,6 +710,7 @@ def c4_part(host, workloads):
         "autotuned": res.to_json()["best"],
         "search_seconds": tune_s,
         "search_table_ms_on_512_candidate_sample": res.to_json()["table"],
+        "confirm_table_ms_on_whole_batch": res.to_json()["confirm_table"],
     }
 
 

@@ -149,3 +149,7 @@ def test_tune_fit_confirms_a_sample_winner_on_the_whole_batch():
         assert res.best[TUNE_FIT_TILE_ROWS] == want
         assert ev.t[TUNE_FIT_TILE_ROWS] == want
         assert res.builtin_ms == 40.0
+        # The whole-batch timings of both shapes are reported.
+        conf = {s[TUNE_FIT_TILE_ROWS]: ms for s, ms in res.confirm_table}
+        assert conf == {8192: 40.0, 4096: 44.0 if full_prefers_builtin else 36.0}
+        assert len(res.to_json()["confirm_table"]) == 2

@@ -49,6 +49,9 @@ class TuneResult:
     best_ms: float
     builtin_ms: float
     table: List[Tuple[Setting, float]] = field(default_factory=list)
+    # With tune_fit(confirm=True): the sample winner and the built-in shape
+    # timed on the whole batch (empty when the sample kept the built-in).
+    confirm_table: List[Tuple[Setting, float]] = field(default_factory=list)
 
     @property
     def speedup(self) -> float:
@@ -62,6 +65,8 @@ class TuneResult:
             "best_ms": self.best_ms,
             "builtin_ms": self.builtin_ms,
             "table": [[{name[k]: v for k, v in s.items()}, ms] for s, ms in self.table],
+            "confirm_table": [[{name[k]: v for k, v in s.items()}, ms]
+                              for s, ms in self.confirm_table],
         }
 
 
@@ -218,7 +223,7 @@ def tune_fit(ev, plans, iters: int, metric: int = 0, rows: Optional[Sequence[int
                 return ev.fit_batch(plans, iters, metric)[1]
         full = tune(ev, run_all, [dict(BUILTIN_FIT), dict(res.best)], reps=reps, margin=margin)
         if full is not None:
-            res = TuneResult(full.best, full.best_ms, full.builtin_ms, res.table)
+            res = TuneResult(full.best, full.best_ms, full.builtin_ms, res.table, full.table)
     if res is not None and cache is not None:
         cache.put(sig, res.best)
     return res
