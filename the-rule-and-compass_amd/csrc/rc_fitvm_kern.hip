@@ -507,6 +507,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void vm
     else cols = (const T *)(const void *)R_.cols64;
     F U0 = (F)(0), U1 = U0, U2 = U0, U3 = U0, Bd = U0;
     double lsum = 0.0;
+    const bool sq_unit = d.metric == RC_LOSS_SQUARED && R_.w64 == nullptr;
     const int row_end = it.row0 + it.rows;
     for (int base = it.row0; base < row_end; base += 64 * RR) {
         int xrow[RR];
@@ -543,14 +544,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void vm
             wv[r] = R_.w64 ? R_.w64[xrow[r]] : 1.0;
             yv[r] = R_.y64[xrow[r]];
         }
+        if (sq_unit) {
+            /* Squared loss, unit weights: the general form below with
+               wb = 1 (1 * t = t exactly); the square is rounded on its own
+               as there. */
 #pragma unroll
-        for (int r = 0; r < RR; ++r) {
-            const double wb = wv[r];
-            const double rr = (double)pred[r] - yv[r];
-            bool hard = !isfinite(pred[r]) || !isfinite(rr);
-            if (sizeof(T) == 4) hard = hard || fabs((double)pred[r]) > 1e30;
-            const double term = hard ? 1e30 : wb * rc_row_stat((double)pred[r], yv[r], d.metric);
-            lsum += (live[r] && wb > 0.0) ? term : 0.0;
+            for (int r = 0; r < RR; ++r) {
+                const double rr = (double)pred[r] - yv[r];
+                bool hard = !isfinite(pred[r]) || !isfinite(rr);
+                if (sizeof(T) == 4) hard = hard || fabs((double)pred[r]) > 1e30;
+                const double term = hard ? 1e30 : __dmul_rn(rr, rr);
+                lsum += live[r] ? term : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < RR; ++r) {
+                const double wb = wv[r];
+                const double rr = (double)pred[r] - yv[r];
+                bool hard = !isfinite(pred[r]) || !isfinite(rr);
+                if (sizeof(T) == 4) hard = hard || fabs((double)pred[r]) > 1e30;
+                const double term = hard ? 1e30 : wb * rc_row_stat((double)pred[r], yv[r], d.metric);
+                lsum += (live[r] && wb > 0.0) ? term : 0.0;
+            }
         }
     }
     const double ls = rc_wave_sum_f64(lsum);
