@@ -306,6 +306,20 @@ __device__ __forceinline__ void ft_tri_index(int t, int &i, int &j)
     j = t - r * (r + 1) / 2;
 }
 
+/* (i, j) of the packed lower-triangle indices t = lane + 64 u, u < 4: every
+   entry of a triangle of up to FT_KMAX rows, worked out once per solve. */
+struct FtTriMap {
+    int i[4], j[4];
+};
+static_assert(FT_KMAX * (FT_KMAX + 1) / 2 <= 4 * 64, "four triangle entries per lane");
+__device__ __forceinline__ FtTriMap ft_tri_map(int lane)
+{
+    FtTriMap m;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ft_tri_index(lane + 64 * u, m.i[u], m.j[u]);
+    return m;
+}
+
 /* One Cholesky factorization of the scaled lower triangle in S (row-major,
    stride FT_KMAX), right-looking over the 64 lanes of a one-wave workgroup:
    at step j the diagonal is checked and square-rooted, column j is divided
@@ -315,7 +329,7 @@ __device__ __forceinline__ void ft_tri_index(int t, int &i, int &j)
    the first row whose diagonal fails (p when none); lane 0 accumulates
    0.5 ln s of the accepted diagonals in row order. */
 template <bool UNDAMPED>
-__device__ int ft_chol_wave(double *S, int p, int lane, double &sum_ln)
+__device__ int ft_chol_wave(double *S, int p, int lane, double &sum_ln, const FtTriMap &tm)
 {
     for (int j = 0; j < p; ++j) {
         const double d = S[j * FT_KMAX + j];
@@ -330,10 +344,11 @@ __device__ int ft_chol_wave(double *S, int p, int lane, double &sum_ln)
         for (int i = j + 1 + lane; i < p; i += 64) S[i * FT_KMAX + j] = S[i * FT_KMAX + j] / ljj;
         __syncthreads();
         const int n = p - j - 1;
-        for (int t = lane; t < n * (n + 1) / 2; t += 64) {
-            int a, b;
-            ft_tri_index(t, a, b);
-            const int i = j + 1 + a, m = j + 1 + b;
+        const int nt = n * (n + 1) / 2;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (lane + 64 * u >= nt) break;
+            const int i = j + 1 + tm.i[u], m = j + 1 + tm.j[u];
             S[i * FT_KMAX + m] = fma(-S[i * FT_KMAX + j], S[m * FT_KMAX + j], S[i * FT_KMAX + m]);
         }
         __syncthreads();
@@ -367,10 +382,13 @@ __device__ int ft_solve_wave(const double *jtj, const double *jtr, double lambda
     if (__any(bad)) return 1;
     __syncthreads();
     const int ntri = p * (p + 1) / 2;
+    const FtTriMap tm = ft_tri_map(lane);
     unsigned nfrow = 0; /* rows holding a non-finite scaled entry */
-    for (int t = lane; t < ntri; t += 64) {
-        int i, j;
-        ft_tri_index(t, i, j);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int t = lane + 64 * u;
+        if (t >= ntri) break;
+        const int i = tm.i[u], j = tm.j[u];
         const double s = jtj[t] / (dsc[i] * dsc[j]);
         if (!isfinite(s)) nfrow |= 1u << i;
         S[i * FT_KMAX + j] = s;
@@ -378,7 +396,7 @@ __device__ int ft_solve_wave(const double *jtj, const double *jtr, double lambda
     for (int off = 32; off > 0; off >>= 1) nfrow |= (unsigned)__shfl_xor((int)nfrow, off, 64);
     __syncthreads();
     double sum_ln = 0.0;
-    const int fail = ft_chol_wave<true>(S, p, lane, sum_ln);
+    const int fail = ft_chol_wave<true>(S, p, lane, sum_ln, tm);
     const unsigned reached = fail >= p ? 0xffffffffu : (fail >= 31 ? 0xffffffffu : ((2u << fail) - 1u));
     if (nfrow & reached) return 1;
     int rank_deficient = 1;
@@ -387,16 +405,18 @@ __device__ int ft_solve_wave(const double *jtj, const double *jtr, double lambda
         rank_deficient = (!isfinite(ln_det) || ln_det < FT_LN_DET_FLOOR) ? 1 : 0;
     }
     __syncthreads();
-    for (int t = lane; t < ntri; t += 64) {
-        int i, j;
-        ft_tri_index(t, i, j);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int t = lane + 64 * u;
+        if (t >= ntri) break;
+        const int i = tm.i[u], j = tm.j[u];
         double s = jtj[t] / (dsc[i] * dsc[j]);
         if (i == j) s = fma(s, 1.0 + lambda, 1e-12);
         S[i * FT_KMAX + j] = s;
     }
     __syncthreads();
     double unused = 0.0;
-    if (ft_chol_wave<false>(S, p, lane, unused) < p) return 1;
+    if (ft_chol_wave<false>(S, p, lane, unused, tm) < p) return 1;
     if (lane == 0) {
         for (int i = 0; i < p; ++i) {
             double s = -jtr[i] / dsc[i];
