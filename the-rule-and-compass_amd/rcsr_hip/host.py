@@ -153,7 +153,7 @@ def fit_status_usable(status: int) -> bool:
     return status in (0, 4, 5, 6, 7, 8)
 
 
-@dataclass
+@dataclass(slots=True)
 class GpuFitOutput:
     """gpu.rs:828."""
     c: np.ndarray
@@ -161,16 +161,29 @@ class GpuFitOutput:
     status: int
 
 
+def _fit_outputs(plans, outc, outsse, outst) -> List[GpuFitOutput]:
+    """One output per plan: its constants (a view of its row of outc), loss
+    and status."""
+    rows = list(outc)
+    sse = outsse.tolist()
+    st = outst.tolist()
+    return [GpuFitOutput(rows[i][:p.nconst], sse[i], st[i]) for i, p in enumerate(plans)]
+
+
 def pack_plans(plans: Sequence[GpuFitPlan], kmax: Optional[int] = None):
-    kmax = kmax or max(1, max(p.nconst for p in plans))
-    words = np.concatenate([p.words for p in plans]).astype(np.uint32)
-    off = np.zeros(len(plans) + 1, np.uint32)
-    off[1:] = np.cumsum([len(p.words) for p in plans])
-    pcount = np.asarray([p.nconst for p in plans], np.int32)
-    c0 = np.zeros((len(plans), kmax))
-    for i, p in enumerate(plans):
-        c0[i, :p.nconst] = p.c0
-    return np.ascontiguousarray(words), off, pcount, np.ascontiguousarray(c0), kmax
+    """The batch arrays of rc_fit_batch: concatenated records, their word
+    offsets, constant counts and the start constants padded to kmax."""
+    n = len(plans)
+    pcount = np.fromiter((p.nconst for p in plans), np.int32, count=n)
+    kmax = kmax or max(1, int(pcount.max()))
+    ws = [p.words for p in plans]
+    words = np.concatenate(ws).astype(np.uint32, copy=False)
+    off = np.zeros(n + 1, np.uint32)
+    off[1:] = np.cumsum(np.fromiter((len(w) for w in ws), np.int64, count=n))
+    c0 = np.zeros((n, kmax))
+    if int(pcount.sum()) > 0:
+        c0[np.arange(kmax)[None, :] < pcount[:, None]] = np.concatenate([p.c0 for p in plans])
+    return np.ascontiguousarray(words), off, pcount, c0, kmax
 
 
 class GpuEvaluator:
@@ -331,9 +344,7 @@ class GpuEvaluator:
                                   _L.ptr(gauge, _L.i32p), int(iters), int(metric),
                                   _L.ptr(outc, _L.f64p), _L.ptr(outsse, _L.f64p),
                                   _L.ptr(outst, _L.i32p))
-        outs = [GpuFitOutput(outc[i, :plans[i].nconst].copy(), float(outsse[i]), int(outst[i]))
-                for i in range(n)]
-        return outs, ms
+        return _fit_outputs(plans, outc, outsse, outst), ms
 
     # -- exact order, rank and maximum statistics ------------------------
     def _stage_pack(self, plans, c, ntrials):
@@ -408,9 +419,7 @@ class GpuEvaluator:
                                          _L.ptr(rs, _L.i32p), _L.ptr(mt, _L.i32p), int(iters),
                                          _L.ptr(outc, _L.f64p), _L.ptr(outsse, _L.f64p),
                                          _L.ptr(outst, _L.i32p))
-        outs = [GpuFitOutput(outc[i, :plans[i].nconst].copy(), float(outsse[i]), int(outst[i]))
-                for i in range(n)]
-        return outs, ms
+        return _fit_outputs(plans, outc, outsse, outst), ms
 
     def fit_jacobian(self, plans, c, cand: int, rows: int):
         words, off, pcount, c0p, kmax = pack_plans(plans)
@@ -446,9 +455,7 @@ class GpuEvaluator:
                                            _L.ptr(pcount, _L.i32p), n, kmax, _L.ptr(c0p, _L.f64p),
                                            int(iters), _L.ptr(outc, _L.f64p),
                                            _L.ptr(outsse, _L.f64p), _L.ptr(outst, _L.i32p))
-        outs = [GpuFitOutput(outc[i, :plans[i].nconst].copy(), float(outsse[i]), int(outst[i]))
-                for i in range(n)]
-        return outs, ms
+        return _fit_outputs(plans, outc, outsse, outst), ms
 
     def stats(self) -> "_L.RcStats":
         s = _L.RcStats()
