@@ -337,6 +337,7 @@ def main():
     # squared / L1): the built-in launch shapes against the autotuned ones.
     if not args.no_c4:
         result["c4_shapes"] = c4_part(host, workloads)
+        result["small_batch_shapes"] = small_batch_part(host, workloads)
     # Exact order / rank / maximum stages on the C3 rows.
     if not args.no_order:
         result["order_stages"] = order_part(host, workloads)
@@ -652,6 +653,47 @@ def order_part(host, workloads, ncand=64, ntrials=4):
     log("order stages: " + ", ".join(f"{k} {v['ms']:.1f} ms" for k, v in out.items()
                                      if isinstance(v, dict)))
     return out
+
+
+def small_batch_part(host, workloads, ncand=48):
+    """A batch too small to fill the GPU with the built-in work items: the
+    first 48 C3 candidates (N = 262144, 30 LM iterations, 32 items of 8192
+    rows each), built-in shapes against the autotuner's choice, timed on the
+    whole batch (best of 2 fits each)."""
+    import numpy as np
+    from rcsr_hip import autotune
+    wl = workloads.config3(n=262144, ncand=ncand, iters=30)
+    plans = [host.gpu_fit_plan(t) for t in wl.trees]
+    ev = host.GpuEvaluator(wl.cols32(), wl.n, wl.v, wl.y.astype(np.float32))
+    assert ev.set_fit_data(wl.cols64(), wl.y)
+
+    def fit_ms():
+        best = float("inf")
+        for _ in range(2):
+            outs, ms = ev.fit_batch(plans, 30, 0, kmax=20)
+            if ms < 0:
+                raise RuntimeError("rc_fit_batch failed")
+            best = min(best, ms)
+        return best, outs
+    fixed, o_fixed = fit_ms()
+    ts = time.perf_counter()
+    res = autotune.tune_fit(ev, plans, 30, rows=[wl.n] * len(plans), reps=1)
+    tune_s = time.perf_counter() - ts
+    tuned, o_tuned = fit_ms()
+    ev.close()
+    agree = float(np.mean([abs(a.sse - b.sse) <= 1e-6 * abs(b.sse) for a, b in zip(o_tuned, o_fixed)]))
+    log(f"small batch ({ncand} C3 candidates): built-in shapes {fixed:.1f} ms, autotuned "
+        f"{tuned:.1f} ms ({res.to_json()['best']}, search {tune_s:.1f}s)")
+    return {
+        "workload": f"first {ncand} C3 candidates, N={wl.n}, 30 LM iterations",
+        "fixed_ms": fixed,
+        "autotuned_ms": tuned,
+        "speedup": fixed / tuned,
+        "autotuned": res.to_json()["best"],
+        "search_seconds": tune_s,
+        "search_table_ms": res.to_json()["table"],
+        "losses_within_1e-6_of_built_in": agree,
+    }
 
 
 def c4_part(host, workloads):
