@@ -397,23 +397,37 @@ __device__ __noinline__ ImUnary im_unary_all(uint32_t op, double a, double ga)
 {
     ImUnary u;
     u.v = im_value(op, a, 0.0);
-    u.g = im_guarded(op, ga, 0.0);
+    /* The guarded evaluation runs the value's function on its own argument:
+       when that is a itself (bit for bit) the result is the value, through
+       the guarded operators' domain checks. */
+    if (__double_as_longlong(ga) == __double_as_longlong(a)) {
+        switch (op) {
+            case OP_LOG: u.g = a > 0.0 ? u.v : RC_F64_NAN; break;
+            case OP_SQRT: u.g = a >= 0.0 ? u.v : RC_F64_NAN; break;
+            case OP_TAN: u.g = isfinite(u.v) ? u.v : RC_F64_NAN; break;
+            default: u.g = u.v; break;
+        }
+    } else {
+        u.g = im_guarded(op, ga, 0.0);
+    }
     u.guard = im_guard(op, a, 0.0, u.v) ? 1 : 0;
     u.div = 0;
+    /* A partial that is the value itself, or a multiple of it, reuses it (the
+       same expression of the same argument). */
     switch (op) {
         case OP_NEG: u.p = -1.0; break;
-        case OP_EXP: u.p = exp(a); break;
+        case OP_EXP: u.p = a > 700.0 ? exp(a) : u.v; break;
         case OP_LOG: u.p = a; u.div = 1; break;
         case OP_ABS: u.p = a >= 0.0 ? 1.0 : -1.0; break;
-        case OP_SQRT: u.p = 2.0 * sqrt(a); u.div = 1; break;
+        case OP_SQRT: u.p = 2.0 * u.v; u.div = 1; break;
         case OP_SIN: u.p = cos(a); break;
         case OP_COS: u.p = -sin(a); break;
         case OP_TAN: { const double c = cos(a); u.p = c * c; u.div = 1; } break;
         case OP_SINH: u.p = cosh(a); break;
         case OP_COSH: u.p = sinh(a); break;
-        case OP_TANH: { const double h = tanh(a); u.p = 1.0 - h * h; } break;
-        case OP_SIG: { const double e = 1.0 / (1.0 + exp(-a)); u.p = e * (1.0 - e); } break;
-        case OP_GAUSS: u.p = (-2.0 * a) * exp(-a * a); break;
+        case OP_TANH: { const double h = u.v; u.p = 1.0 - h * h; } break;
+        case OP_SIG: { const double e = u.v; u.p = e * (1.0 - e); } break;
+        case OP_GAUSS: u.p = (-2.0 * a) * u.v; break;
         case OP_SQUARE: u.p = 2.0 * a; break;
         case OP_CUBE: u.p = 3.0 * a * a; break;
         case OP_ERF: u.p = 2.0 * IM_INV_SQRT_PI * exp(-a * a); break;
